@@ -1,0 +1,93 @@
+// AlgorithmHandler on the MI355X GPU library (include/lbm_hip.h).  Replaces
+// the reference's SYCLAlgorithmHandler + SYCLAlgorithm + NPTL/LPTL/NPOL/NSOL
+// drivers (include/lbm/execution/sycl_algorithm_handler.hpp:45-455,
+// sycl_algorithm.hpp:31-116, gpu/*/*.hpp): the same construction checks,
+// lifecycle and accessors; the step loop runs batches of fused steps on a
+// worker thread and copies the macroscopic fields to the host on frame steps
+// (iteration % frameUpdateInterval == frameUpdateInterval - 1) and at the end.
+#ifndef LBM_HIP_ALGORITHM_HANDLER_HPP
+#define LBM_HIP_ALGORITHM_HANDLER_HPP
+
+#include <atomic>
+#include <cstdint>
+#include <future>
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+
+#include "../core/control.hpp"
+#include "../core/domain.hpp"
+#include "../core/properties.hpp"
+#include "algorithm_handler.hpp"
+
+struct lbm_ctx;
+
+namespace lbm
+{
+namespace execution
+{
+
+class HipAlgorithmHandler : public AlgorithmHandler
+{
+  public:
+    // Reads the settings file (sycl_algorithm_handler.hpp:334-346).
+    explicit HipAlgorithmHandler(const std::string &settings_path);
+    // Uses the given properties (sycl_algorithm_handler.hpp:355-368).
+    explicit HipAlgorithmHandler(core::Properties props);
+    ~HipAlgorithmHandler() override;
+
+    HipAlgorithmHandler(const HipAlgorithmHandler &) = delete;
+    HipAlgorithmHandler &operator=(const HipAlgorithmHandler &) = delete;
+
+    void initialize() override;
+    void start() override;
+    void pause() override;
+    bool is_finished() const override;
+    void block_until_finished() override;
+
+    std::vector<real_type> &get_densities() const override { return densities_; }
+    std::vector<real_type> &get_x_velocities() const override { return x_velocities_; }
+    std::vector<real_type> &get_y_velocities() const override { return y_velocities_; }
+    std::vector<real_type> &get_absolute_velocities() const override { return absolute_velocities_; }
+
+    unsigned int get_horizontal_nodes() const override { return properties_->horizontal_nodes; }
+    unsigned int get_vertical_nodes() const override { return properties_->vertical_nodes; }
+    real_type get_inlet_density() const override { return properties_->inlet_density; }
+    real_type get_outlet_density() const override { return properties_->outlet_density; }
+    real_type get_progress() const override { return control_->get_progress(); }
+    real_type get_last_frametime() const override { return control_->get_last_frametime(); }
+    unsigned int get_current_iteration() const override { return control_->get_current_iteration(); }
+    const core::Properties &get_properties() const override { return *properties_; }
+    const core::Domain &get_domain() const override { return *domain_; }
+    const std::string &get_settings_path() const { return settings_path_; }
+
+    // Beyond the reference interface: the node-type table handed to the GPU,
+    // the current populations (9 * expanded node count, layout order) and the
+    // underlying C-ABI context.  Not to be used while the worker runs.
+    const std::vector<std::int8_t> &get_phase() const { return phase_; }
+    std::vector<double> get_populations() const;
+    lbm_ctx *context() const { return ctx_; }
+
+  private:
+    void initialize_algorithm();
+    void release();
+    void execute();  // worker body
+    void copy_macroscopic_observables_to_cpu();
+
+    std::string settings_path_;
+    std::optional<core::Properties> explicit_props_;
+    std::unique_ptr<core::Properties> properties_;
+    std::unique_ptr<core::Domain> domain_;
+    std::unique_ptr<core::Control> control_;
+    std::vector<std::int8_t> phase_;
+    mutable std::vector<real_type> densities_, x_velocities_, y_velocities_, absolute_velocities_;
+    lbm_ctx *ctx_ = nullptr;
+    std::future<void> future_;
+    bool active_ = false;
+};
+
+}  // namespace execution
+}  // namespace lbm
+
+#endif  // LBM_HIP_ALGORITHM_HANDLER_HPP

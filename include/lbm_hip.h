@@ -1,0 +1,199 @@
+/*
+ * lbm_hip.h -- C ABI of the MI355X (gfx950) D2Q9 BGK time step.
+ *
+ * This is the kernel-level boundary of the library: plain C types, plain
+ * pointers and sizes, status codes instead of exceptions.  It replaces the
+ * reference's SYCL functors and the device-memory owners they read from:
+ *
+ *   reference (SC-SGS/PortLBM)                           replaced by
+ *   ---------------------------------------------------  ------------------------------
+ *   core::Simulation / Data / Results ctors              lbm_create
+ *     src/lbm/core/simulation.cpp:386-433,482-519
+ *   device max_work_group_size query                     lbm_device_max_work_group_size
+ *     src/lbm/core/simulation.cpp:487-499
+ *   phase fills, set_pipe_boundaries, add_stencil,       lbm_set_phase
+ *   porous H2D copy
+ *     include/lbm/core/domain_initialization.hpp:214-239,414-518
+ *   set_standstill_values, set_inout_distribution_values lbm_init_populations
+ *   and the lattice 0 -> 1 copy
+ *     include/lbm/core/domain_initialization.hpp:311-403,460-467
+ *   NPTL/LPTL/NPOL/NSOL<A>::execute() step loop          lbm_run
+ *     include/lbm/gpu/nptl/nptl.hpp:116-141, lptl/lptl.hpp:112-137,
+ *     npol/npol.hpp:180-201, nsol/nsol.hpp:184-205
+ *     (EmplaceBounceBackKernel, OutletUpdateKernel, StreamCollideKernel and
+ *      the buffer-copy kernels of every algorithm are fused into one launch
+ *      per step plus, for the one-lattice schemes, one halo refresh)
+ *   SYCLAlgorithm::copy_macroscopic_observables_to_cpu   lbm_get_macros
+ *     include/lbm/execution/sycl_algorithm.hpp:90-104
+ *   Data::distribution_values_0 (device lattice)         lbm_get_populations /
+ *     include/lbm/core/simulation.hpp:240-254              lbm_set_populations
+ *   Data::phase_information                              lbm_get_phase
+ *   get_neighbor + EmplaceBounceBackKernel addressing    lbm_get_node_codes
+ *     include/lbm/core/access.hpp:203-213,
+ *     include/lbm/gpu/nptl/nptl_kernels.hpp:471-491
+ *   queue->wait()                                        lbm_sync
+ *   exceptions crossing the API                          int status + lbm_last_error
+ *
+ * The functions below that have no reference counterpart (precision storage,
+ * FP64 reductions, residual, batched domains) say so in their comments.
+ *
+ * Threading: one lbm_ctx is used by one host thread at a time.  Every call
+ * runs on the context's own HIP stream (default priority).
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LBM_OK 0
+#define LBM_ERR_INVALID_ARGUMENT 1 /* reference: exceptions::json::PropertyArgumentException */
+#define LBM_ERR_DEVICE 2           /* a HIP runtime call failed */
+#define LBM_ERR_OUT_OF_MEMORY 3
+#define LBM_ERR_STATE 4            /* call out of order (e.g. run before init) */
+
+/* ---- enumerations ------------------------------------------------------ */
+/* core/constants.hpp:43-48 */
+enum lbm_algorithm { LBM_ALG_NPTL = 0, LBM_ALG_LPTL = 1, LBM_ALG_NPOL = 2, LBM_ALG_NSOL = 3 };
+/* core/constants.hpp:50, core/access.hpp:102-172 */
+enum lbm_layout { LBM_LAYOUT_COLLISION = 0, LBM_LAYOUT_STREAM = 1, LBM_LAYOUT_BUNDLE = 2 };
+/* Population storage type.  The reference has only the compile-time
+ * real_type switch (core/constants.hpp:17-21, double or float).  F16/BF16
+ * store the deviation f_i - w_i; collision arithmetic then runs in FP32. */
+enum lbm_storage {
+    LBM_STORE_F64 = 0,
+    LBM_STORE_F32 = 1,
+    LBM_STORE_F16_SHIFTED = 2,
+    LBM_STORE_BF16_SHIFTED = 3
+};
+
+/* Node-code bits returned by lbm_get_node_codes (one uint16 per expanded node).
+ * Bit d (0..8) set: the population pulled along direction d comes from a
+ * solid node, so half-way bounce-back supplies the node's own post-collision
+ * population 8-d (nptl_kernels.hpp:471-491 emplaces exactly that value). */
+#define LBM_CODE_BOUNCE_MASK 0x01FFu
+#define LBM_CODE_FLUID 0x0200u      /* phase == 0: the node is updated */
+#define LBM_CODE_OUTLET 0x0400u     /* Zou-He pressure outlet node (x = H-2) */
+#define LBM_CODE_UY_SOURCE 0x0800u  /* x = H-3: its uy feeds the outlet next step */
+#define LBM_CODE_SOLID 0x1000u      /* phase == 1 */
+
+/* ---- lattice description ----------------------------------------------- */
+/* Mirrors core::Properties (core/properties.hpp:30-166) and core::Domain
+ * (core/domain.hpp:29-85) after the reference's own extent arithmetic
+ * (src/lbm/core/simulation.cpp:194-382).  horizontal/vertical_nodes include
+ * the two ghost layers; the expanded_* / subdomain_* fields are Domain's. */
+typedef struct lbm_desc {
+    int32_t algorithm;      /* enum lbm_algorithm */
+    int32_t layout;         /* enum lbm_layout */
+    int32_t storage;        /* enum lbm_storage */
+    int32_t nan_protection; /* 1: NaN or > FLT_MAX outputs are written as 0 (WITH_NAN_PROTECTION) */
+    uint32_t horizontal_nodes;
+    uint32_t vertical_nodes;
+    uint32_t expanded_horizontal_nodes;
+    uint32_t expanded_vertical_nodes;
+    uint32_t subdomain_horizontal_nodes;
+    uint32_t subdomain_vertical_nodes;
+    uint32_t subdomain_count_horizontal;
+    uint32_t subdomain_count_vertical;
+    uint32_t work_group_size;
+    uint32_t reserved0;
+    double relaxation_time;
+    double inlet_density;
+    double inlet_velocity_x;
+    double inlet_velocity_y;
+    double outlet_density;
+    double outlet_velocity_x;
+    double outlet_velocity_y;
+} lbm_desc;
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Thread-local text of the last error of the calling thread ("" if none). */
+const char *lbm_last_error(void);
+
+/* Largest work-group the device accepts (1024 on gfx950).
+ * Replaces sycl::info::device::max_work_group_size (simulation.cpp:443,487). */
+int lbm_device_max_work_group_size(uint32_t *out);
+
+/* Allocate the lattice(s), phase/node-code tables and result arrays.
+ * Two lattices for nptl/lptl, one for npol/nsol (simulation.cpp:464-479).
+ * Results start at density -1 and zero momentum (simulation.cpp:386-417). */
+int lbm_create(const lbm_desc *desc, lbm_ctx **out);
+void lbm_destroy(lbm_ctx *ctx);
+
+/* Upload the node-type (phase) table: one int8 per expanded node,
+ * -1 ghost/buffer/padding, 0 fluid, 1 solid (domain_initialization.hpp:453-518). */
+int lbm_set_phase(lbm_ctx *ctx, const int8_t *phase, size_t node_count);
+int lbm_get_phase(lbm_ctx *ctx, int8_t *phase, size_t node_count);
+
+/* Rest equilibrium everywhere, inlet/outlet ghost columns at their
+ * equilibria, copy to the second lattice, derive the node codes from the
+ * phase table, zero the outlet's uy history and the step counter.
+ * Requires lbm_set_phase first. */
+int lbm_init_populations(lbm_ctx *ctx);
+
+/* Node codes (LBM_CODE_* bits), one uint16 per expanded node. */
+int lbm_get_node_codes(lbm_ctx *ctx, uint16_t *codes, size_t node_count);
+
+/* Advance `steps` time steps.  Macroscopic outputs (density, momentum,
+ * |momentum|) are written on the device on frame steps -- the global
+ * iteration it with it % frame_interval == frame_interval - 1
+ * (nptl.hpp:71-75) -- and on the last step of this call, which is when the
+ * reference copies them to the host.  The outlet's uy input is kept live on
+ * every step in its own column.  frame_interval 0 means "last step only". */
+int lbm_run(lbm_ctx *ctx, uint32_t steps, uint32_t frame_interval);
+
+/* Steps completed since lbm_init_populations. */
+int lbm_get_iteration(lbm_ctx *ctx, uint64_t *iteration);
+
+/* Device time of the last lbm_run call in milliseconds, measured with HIP
+ * events recorded on the context's stream around its launches. */
+int lbm_last_run_ms(lbm_ctx *ctx, double *ms);
+
+/* Copy the four result arrays, each (H-2)*(V-2) values in row-major inner
+ * order r = (x-1) + (y-1)*(H-2) (core/access.hpp:269-273).  Any pointer may
+ * be NULL to skip that array. */
+int lbm_get_macros(lbm_ctx *ctx, double *density, double *x_velocity, double *y_velocity,
+                   double *absolute_velocity, size_t count);
+
+/* Current source lattice: 9 * expanded node count values in the context's
+ * layout order (core/access.hpp:117-171), widened to double; shifted
+ * storage is returned as f_i (the weight added back). */
+int lbm_get_populations(lbm_ctx *ctx, double *values, size_t count);
+int lbm_set_populations(lbm_ctx *ctx, const double *values, size_t count);
+
+/* Overwrite every fluid node with the equilibrium of a given (density,
+ * momentum) field given on the inner grid ((H-2)*(V-2) values each, same
+ * order as lbm_get_macros).  Used by warm starts.  No reference counterpart. */
+int lbm_set_equilibrium_field(lbm_ctx *ctx, const double *density, const double *x_velocity,
+                              const double *y_velocity, size_t count);
+
+/* FP64 reductions over fluid nodes of the current source lattice, computed
+ * on the device with wave64 shuffles: out[0] = sum rho, out[1] = sum jx,
+ * out[2] = sum jy, out[3] = fluid node count, out[4], out[5] = momentum-
+ * exchange force (x, y) on the solid links.  No reference counterpart. */
+int lbm_reduce(lbm_ctx *ctx, double out[6]);
+
+/* Steady-state residual ||j^now - j^snap||_2 / ||j^now||_2 over fluid
+ * nodes (FP64 on the device), then j^snap := j^now.  The first call after
+ * init compares against zero and returns 1.  No reference counterpart. */
+int lbm_residual(lbm_ctx *ctx, double *residual);
+
+/* Block until all work on the context's stream is done (queue->wait()). */
+int lbm_sync(lbm_ctx *ctx);
+
+/* Device-to-device copy bandwidth of `bytes` bytes, best of `repeats`
+ * (GB/s, read+write counted).  The measured ceiling beside the nominal
+ * 8 TB/s HBM figure.  No reference counterpart. */
+int lbm_measure_copy_bandwidth(size_t bytes, int repeats, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,168 @@
+"""ctypes binding of oracle/liboracle.so (test infrastructure only).
+
+The oracle restates the reference's algorithm pass by pass in C (oracle.c);
+this module only marshals arrays.  Scenario ids follow oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+ALGORITHMS = {"nptl": 0, "lptl": 1, "npol": 2, "nsol": 3}
+LAYOUTS = {"collision": 0, "stream": 1, "bundle": 2}
+SCENARIOS = {"Hagen-Poiseuille": 0, "walls": 1, "circle": 2, "square": 3, "wing": 4, "skyscraper": 5,
+             "porous": 6, "plate": 7, "porous-circles": 8, "cylinder": 9}
+
+_lib: ctypes.CDLL | None = None
+
+
+class ScenarioParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("porosity", ctypes.c_double), ("radius_min", ctypes.c_double),
+                ("radius_max", ctypes.c_double), ("margin", ctypes.c_uint), ("cylinder_x", ctypes.c_uint),
+                ("cylinder_radius", ctypes.c_double)]
+
+
+def library() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run `make -C oracle` (or portlbm_amd.build)")
+        lib = ctypes.CDLL(str(path))
+        P, U, D = ctypes.c_void_p, ctypes.c_uint, ctypes.c_double
+        lib.oracle_domain.argtypes = [ctypes.c_int, U, U, U, P]
+        lib.oracle_domain.restype = ctypes.c_int
+        lib.oracle_glibc_rand_seed1.argtypes = [P, ctypes.c_size_t]
+        lib.oracle_create.argtypes = [ctypes.c_int, ctypes.c_int, U, U, U, D, D, D, D, D, D, D, ctypes.c_int]
+        lib.oracle_create.restype = P
+        lib.oracle_destroy.argtypes = [P]
+        lib.oracle_build_phase.argtypes = [P, ctypes.c_int, ctypes.POINTER(ScenarioParams)]
+        lib.oracle_build_phase.restype = ctypes.c_int
+        lib.oracle_set_phase.argtypes = [P, P, ctypes.c_size_t]
+        lib.oracle_set_phase.restype = ctypes.c_int
+        lib.oracle_phase.argtypes = [P]
+        lib.oracle_phase.restype = P
+        lib.oracle_node_count.argtypes = [P]
+        lib.oracle_node_count.restype = ctypes.c_size_t
+        lib.oracle_result_count.argtypes = [P]
+        lib.oracle_result_count.restype = ctypes.c_size_t
+        lib.oracle_extents.argtypes = [P, P]
+        lib.oracle_init.argtypes = [P]
+        lib.oracle_run.argtypes = [P, U]
+        lib.oracle_populations.argtypes = [P]
+        lib.oracle_populations.restype = P
+        lib.oracle_set_populations.argtypes = [P, P, ctypes.c_size_t]
+        lib.oracle_results.argtypes = [P, P, P, P, P]
+        lib.oracle_node_codes.argtypes = [P, P]
+        lib.oracle_reduce.argtypes = [P, P]
+        lib.oracle_set_threads.argtypes = [ctypes.c_int]
+        lib.oracle_max_threads.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def domain(algorithm: str, H: int, V: int, wg: int) -> dict | None:
+    out = np.zeros(7, dtype=np.uint32)
+    if library().oracle_domain(ALGORITHMS[algorithm], H, V, wg, _p(out)) != 0:
+        return None
+    return dict(zip(("Hx", "Vx", "sub_h", "sub_v", "cnt_h", "cnt_v", "N"), map(int, out)))
+
+
+def glibc_rand_seed1(count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.int32)
+    library().oracle_glibc_rand_seed1(_p(out), count)
+    return out
+
+
+def set_threads(n: int) -> None:
+    library().oracle_set_threads(n)
+
+
+def max_threads() -> int:
+    return library().oracle_max_threads()
+
+
+class Simulation:
+    """The reference's simulation of one settings document, on the CPU."""
+
+    def __init__(self, doc: dict):
+        lib = library()
+        alg, dom, phy = doc["algorithmic"], doc["domain"], doc["physical"]
+        ext = doc.get("mi355x", {})
+        H, V = dom["horizontalNodes"] + 2, dom["verticalNodes"] + 2
+        self.H, self.V = H, V
+        self._s = lib.oracle_create(ALGORITHMS[alg["algorithm"]], LAYOUTS[alg["dataLayout"]], H, V,
+                                    alg["workGroupSize"], phy["relaxationTime"], phy["inletDensity"],
+                                    phy["inletVelocity"]["x"], phy["inletVelocity"]["y"], phy["outletDensity"],
+                                    phy["outletVelocity"]["x"], phy["outletVelocity"]["y"],
+                                    1 if ext.get("nanProtection", True) else 0)
+        if not self._s:
+            raise ValueError("oracle_create rejected the settings")
+        e = np.zeros(7, dtype=np.uint32)
+        lib.oracle_extents(self._s, _p(e))
+        self.extents = dict(zip(("Hx", "Vx", "sub_h", "sub_v", "cnt_h", "cnt_v", "N"), map(int, e)))
+        self.N = self.extents["N"]
+        self.D = (H - 2) * (V - 2)
+        params = ScenarioParams(int(ext.get("seed", 20261202)), float(ext.get("porosity", 0.7)),
+                                float(ext.get("radiusMin", 4.0)), float(ext.get("radiusMax", 16.0)),
+                                int(ext.get("obstacleMargin", 32)), int(ext.get("cylinderX", 1024)),
+                                float(ext.get("cylinderRadius", 64.0)))
+        if lib.oracle_build_phase(self._s, SCENARIOS[dom["scenario"]], ctypes.byref(params)) != 0:
+            raise ValueError("oracle_build_phase failed")
+        lib.oracle_init(self._s)
+
+    def close(self) -> None:
+        if self._s:
+            library().oracle_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def phase(self) -> np.ndarray:
+        ptr = library().oracle_phase(self._s)
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int8)), (self.N,)).copy()
+
+    def set_phase(self, phase: np.ndarray) -> None:
+        phase = np.ascontiguousarray(phase, dtype=np.int8)
+        if library().oracle_set_phase(self._s, _p(phase), phase.size) != 0:
+            raise ValueError("phase size mismatch")
+
+    def init(self) -> None:
+        library().oracle_init(self._s)
+
+    def run(self, steps: int) -> None:
+        library().oracle_run(self._s, steps)
+
+    def populations(self) -> np.ndarray:
+        ptr = library().oracle_populations(self._s)
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)), (9 * self.N,)).copy()
+
+    def set_populations(self, values: np.ndarray) -> None:
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        library().oracle_set_populations(self._s, _p(values), values.size)
+
+    def results(self) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+        out = [np.empty(self.D) for _ in range(4)]
+        library().oracle_results(self._s, *map(_p, out))
+        return tuple(out)  # type: ignore[return-value]
+
+    def node_codes(self) -> np.ndarray:
+        out = np.empty(self.N, dtype=np.uint16)
+        library().oracle_node_codes(self._s, _p(out))
+        return out
+
+    def reduce(self) -> dict:
+        out = np.zeros(6)
+        library().oracle_reduce(self._s, _p(out))
+        return {"mass": out[0], "jx": out[1], "jy": out[2], "fluid": int(out[3]), "fx": out[4], "fy": out[5]}

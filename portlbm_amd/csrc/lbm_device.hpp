@@ -1,0 +1,243 @@
+// Device code shared by every D2Q9 BGK kernel of the library: population
+// layouts, storage types, the pull with fused half-way bounce-back, the
+// fused Zou-He pressure outlet and the BGK collision.
+//
+// Arithmetic follows the reference kernel text term by term so that the FP64
+// path reproduces it bit for bit when no contraction is applied (the library
+// is compiled with -ffp-contract=off):
+//   moments + collision: include/lbm/gpu/nptl/nptl_kernels.hpp:371-437
+//   outlet:              include/lbm/gpu/general/non_buffered.hpp:75-130
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lbm_hip.h"
+
+namespace lbm_dev {
+
+// ---- D2Q9 lattice (core/constants.hpp:31-39, core/access.hpp:203-213) ----
+// direction d: cx = d % 3 - 1, cy = d / 3 - 1, opposite = 8 - d.
+__device__ __forceinline__ constexpr int cx_of(int d) { return d % 3 - 1; }
+__device__ __forceinline__ constexpr int cy_of(int d) { return d / 3 - 1; }
+
+template <class C>
+__device__ __forceinline__ C weight(int d)
+{
+    // 1/36, 1/9, 1/36, 1/9, 4/9, 1/9, 1/36, 1/9, 1/36 (constants.hpp:38-39)
+    return (d == 4) ? C(4.0 / 9) : ((d & 1) ? C(1.0 / 9) : C(1.0 / 36));
+}
+
+// ---- population layouts (core/access.hpp:117-171) ----------------------
+template <int L>
+__device__ __forceinline__ uint32_t at(uint32_t node, uint32_t d, uint32_t total)
+{
+    if constexpr (L == LBM_LAYOUT_COLLISION)
+        return 9u * node + d;
+    else if constexpr (L == LBM_LAYOUT_STREAM)
+        return total * d + node;
+    else
+        return 3u * (d / 3u) * total + (d % 3u) + 3u * node;
+}
+
+// ---- storage types ------------------------------------------------------
+// T: what lives in HBM; C: the arithmetic type of the collision.
+template <int S>
+struct Storage;
+
+template <>
+struct Storage<LBM_STORE_F64>
+{
+    using T = double;
+    using C = double;
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
+    static __device__ __forceinline__ T encode(double v, int) { return v; }
+    static __device__ __forceinline__ double decode(T v, int) { return v; }
+};
+
+template <>
+struct Storage<LBM_STORE_F32>
+{
+    using T = float;
+    using C = float;
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
+    static __device__ __forceinline__ T encode(double v, int) { return float(v); }
+    static __device__ __forceinline__ double decode(T v, int) { return double(v); }
+};
+
+// FP16 deviation storage: HBM holds f_i - w_i as IEEE binary16 (RNE).
+template <>
+struct Storage<LBM_STORE_F16_SHIFTED>
+{
+    using T = _Float16;
+    using C = float;
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
+    {
+        return float(p[i]) + weight<float>(d);
+    }
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int d)
+    {
+        p[i] = _Float16(v - weight<float>(d));
+    }
+    static __device__ __forceinline__ T encode(double v, int d) { return _Float16(float(v - weight<double>(d))); }
+    static __device__ __forceinline__ double decode(T v, int d) { return double(float(v)) + weight<double>(d); }
+};
+
+// bfloat16 bit conversions with round-to-nearest-even.
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        return uint16_t((u >> 16) | 0x40u);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+// BF16 deviation storage: HBM holds f_i - w_i as bfloat16 (RNE).
+template <>
+struct Storage<LBM_STORE_BF16_SHIFTED>
+{
+    using T = uint16_t;
+    using C = float;
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
+    {
+        return bf16_to_f32(p[i]) + weight<float>(d);
+    }
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int d)
+    {
+        p[i] = f32_to_bf16(v - weight<float>(d));
+    }
+    static __device__ __forceinline__ T encode(double v, int d) { return f32_to_bf16(float(v - weight<double>(d))); }
+    static __device__ __forceinline__ double decode(T v, int d) { return double(bf16_to_f32(v)) + weight<double>(d); }
+};
+
+// ---- step parameters ------------------------------------------------------
+struct StepParams
+{
+    const void *src;  // source lattice
+    void *dst;        // destination lattice (== src for one-lattice schemes)
+    const uint16_t *code;
+    double *rho;      // results, (H-2)*(V-2) each
+    double *jx;
+    double *jy;
+    double *jabs;
+    const double *uy_in;  // outlet uy history, read (indexed by unexpanded y)
+    double *uy_out;       // outlet uy history, written this step
+    double omega;         // 1 / tau, computed on the host in double
+    double rho_out;
+    uint32_t H, V;        // unexpanded extents incl. ghosts
+    uint32_t Hx, Vx;      // expanded extents
+    uint32_t N;           // Hx * Vx
+    uint32_t sub_h, sub_v;
+    uint32_t write_macros;
+    uint32_t nan_protection;
+};
+
+// NaN clamp of WITH_NAN_PROTECTION (nptl_kernels.hpp:413-432): NaN or values
+// above FLT_MAX become 0.  Applied to outputs only.
+template <class C>
+__device__ __forceinline__ C clamp_output(C v, uint32_t on)
+{
+    if (on && (__builtin_isnan(v) || double(v) > double(__FLT_MAX__)))
+        return C(0);
+    return v;
+}
+
+// ---- the update of one fluid node ------------------------------------------
+// Pull 9 populations.  Bit d of `code` set: the upstream node n - c_d is solid
+// and half-way bounce-back supplies this node's own population 8-d.
+template <int L, int S>
+__device__ __forceinline__ void pull(const typename Storage<S>::T *src, uint32_t n, uint32_t code,
+                                     uint32_t Hx, uint32_t N, typename Storage<S>::C f[9])
+{
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+    {
+        const bool bb = (code >> d) & 1u;
+        const uint32_t m = bb ? n : uint32_t(int32_t(n) - cy_of(d) * int32_t(Hx) - cx_of(d));
+        const int dd = bb ? 8 - d : d;
+        f[d] = Storage<S>::load(src, at<L>(m, uint32_t(dd), N), dd);
+    }
+}
+
+// Zou-He pressure outlet at x = H-2 (non_buffered.hpp:95-129).  The three
+// populations entering from the ghost column (0, 3, 6) are rebuilt from the
+// six known post-stream populations, the outlet density and the previous
+// step's uy of column H-3.
+template <class C>
+__device__ __forceinline__ void zou_he_outlet(C f[9], C rho_out, C uy_prev)
+{
+    const C jx = -rho_out + (f[1] + f[4] + f[7] + C(2) * (f[8] + f[5] + f[2]));
+    const C f8 = f[8], f5 = f[5], f2 = f[2], f1 = f[1], f7 = f[7];
+    f[0] = f8 - C(0.5) * (f1 - f7) - C(1.0 / 6) * jx - C(0.5) * uy_prev;
+    f[3] = f5 - C(2.0 / 3) * jx;
+    f[6] = f2 + C(0.5) * (f1 - f7) - C(1.0 / 6) * jx + C(0.5) * uy_prev;
+}
+
+// Moments (rho = sum f, j = sum f c) and BGK relaxation towards the
+// incompressible equilibrium feq = w (rho + 3 c.j + 4.5 (c.j)^2 - 1.5 |j|^2);
+// f' = -(1/tau)(f - feq) + f.  Velocity is the momentum j (never divided by
+// rho).  Term order as in nptl_kernels.hpp:382-405.
+template <class C>
+__device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C &usq)
+{
+    rho = C(0);
+    jx = C(0);
+    jy = C(0);
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+    {
+        rho += f[d];
+        jx += f[d] * C(cx_of(d));
+        jy += f[d] * C(cy_of(d));
+    }
+    usq = jx * jx + jy * jy;
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+    {
+        const C cu = C(cx_of(d)) * jx + C(cy_of(d)) * jy;
+        const C feq = weight<C>(d) * (rho + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
+        f[d] = -omega * (f[d] - feq) + f[d];
+    }
+}
+
+template <int L, int S>
+__device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t n, uint32_t N,
+                                           const typename Storage<S>::C f[9])
+{
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+        Storage<S>::store(dst, at<L>(n, uint32_t(d), N), f[d], d);
+}
+
+// Finish one fluid node whose pulled populations are in f: outlet, collision,
+// store, outlet-uy history and (on frame steps) macroscopic outputs at the
+// inner index of unexpanded (x, y).
+template <int L, int S>
+__device__ __forceinline__ void finish_node(const StepParams &p, typename Storage<S>::T *dst, uint32_t n,
+                                            uint32_t code, uint32_t x, uint32_t y,
+                                            typename Storage<S>::C f[9])
+{
+    using C = typename Storage<S>::C;
+    if (code & LBM_CODE_OUTLET)
+        zou_he_outlet<C>(f, C(p.rho_out), C(p.uy_in[y]));
+    C rho, jx, jy, usq;
+    collide<C>(f, C(p.omega), rho, jx, jy, usq);
+    store_node<L, S>(dst, n, p.N, f);
+    if (code & LBM_CODE_UY_SOURCE)
+        p.uy_out[y] = double(clamp_output(jy, p.nan_protection));
+    if (p.write_macros)
+    {
+        const uint32_t r = (x - 1u) + (y - 1u) * (p.H - 2u);
+        const C jabs = __builtin_sqrt(usq);
+        p.rho[r] = double(clamp_output(rho, p.nan_protection));
+        p.jx[r] = double(clamp_output(jx, p.nan_protection));
+        p.jy[r] = double(clamp_output(jy, p.nan_protection));
+        p.jabs[r] = double(clamp_output(jabs, p.nan_protection));
+    }
+}
+
+}  // namespace lbm_dev

@@ -1,0 +1,1152 @@
+// MI355X (gfx950) implementation of the C ABI declared in include/lbm_hip.h.
+//
+// One launch per time step fuses what the reference runs as three to seven
+// blocking launches (include/lbm/gpu/nptl/nptl.hpp:116-141,
+// npol/npol.hpp:180-201, nsol/nsol.hpp:184-205): bounce-back is folded into
+// the pull through the per-node code table, the Zou-He outlet is computed by
+// the outlet nodes themselves, and macroscopic outputs are written only on
+// the steps whose values the host can observe.  The one-lattice schemes keep
+// the reference's buffered geometry and need one halo refresh before each
+// step (two small launches).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_device.hpp"
+
+using namespace lbm_dev;
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define LBM_HIP_TRY(expr)                                                                          \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? LBM_ERR_OUT_OF_MEMORY : LBM_ERR_DEVICE,        \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct lbm_ctx
+{
+    lbm_desc desc;
+    uint32_t H, V, Hx, Vx, N;
+    uint32_t sub_h, sub_v, cnt_h, cnt_v, wg;
+    bool buffered;
+    bool two_lattice;
+    bool outlet_active;
+    size_t elem;          // bytes per stored population
+    void *lat[2] = {nullptr, nullptr};
+    int cur = 0;          // index of the source lattice
+    int8_t *phase = nullptr;
+    uint16_t *code = nullptr;
+    double *res = nullptr;    // 4 * D: rho, jx, jy, |j|
+    double *uycol = nullptr;  // 2 * V outlet uy history (ping-pong)
+    double *snap = nullptr;   // 2 * N residual snapshot (lazy)
+    double *partials = nullptr;
+    double *red_out = nullptr;
+    uint32_t D;               // (H-2)*(V-2)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t iteration = 0;
+    bool phase_set = false;
+    bool initialized = false;
+    bool snap_valid = false;
+    double last_ms = 0.0;
+    double feq_in[9], feq_out[9];
+};
+
+// Host equilibrium with the reference's own expression
+// (core/simulation.hpp:333-350, std::pow on the host).
+static void host_equilibrium(double ux, double uy, double rho, double out[9])
+{
+    static const double w[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9,
+                                1.0 / 9,  1.0 / 36, 1.0 / 9, 1.0 / 36};
+    for (int d = 0; d < 9; ++d)
+    {
+        const int cx = d % 3 - 1, cy = d / 3 - 1;
+        out[d] = w[d] * (rho + 3 * (cx * ux + cy * uy) + 9.0 / 2 * std::pow(cx * ux + cy * uy, 2) -
+                         3.0 / 2 * (ux * ux + uy * uy));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// step kernels
+// ---------------------------------------------------------------------------
+constexpr int NPTL_BLOCK_X = 256;
+
+// nptl: two lattices, 2-D grid over the expanded interior x in [1, Hx-2],
+// y in [1, Vx-2] (nptl.hpp:63-67); expanded and unexpanded coordinates agree.
+template <int L, int S>
+__global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    const uint32_t x = 1u + blockIdx.x * NPTL_BLOCK_X + threadIdx.x;
+    const uint32_t y = 1u + blockIdx.y;
+    if (x > p.Hx - 2u)
+        return;
+    const uint32_t n = x + y * p.Hx;
+    const uint32_t code = p.code[n];
+    if (!(code & LBM_CODE_FLUID))
+        return;
+    C f[9];
+    pull<L, S>(static_cast<const T *>(p.src), n, code, p.Hx, p.N, f);
+    finish_node<L, S>(p, static_cast<T *>(p.dst), n, code, x, y, f);
+}
+
+// lptl: two lattices, 1-D range over every node of the unexpanded lattice
+// (lptl.hpp:63-65), work-group size from the settings.
+template <int L, int S>
+__global__ __launch_bounds__(1024) void k_step_lptl(StepParams p)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.N)
+        return;
+    const uint32_t code = p.code[n];
+    if (!(code & LBM_CODE_FLUID))
+        return;
+    C f[9];
+    pull<L, S>(static_cast<const T *>(p.src), n, code, p.Hx, p.N, f);
+    finish_node<L, S>(p, static_cast<T *>(p.dst), n, code, n % p.H, n / p.H, f);
+}
+
+// npol: one lattice, one work-group per subdomain tile; every pull of the
+// tile is done before the barrier and every write after it, tiles only read
+// their own nodes and the buffer rows/columns around them
+// (npol_kernels.hpp:89-199).
+template <int L, int S>
+__global__ __launch_bounds__(1024) void k_step_npol(StepParams p)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    T *lat = static_cast<T *>(p.dst);
+    const uint32_t x = blockIdx.x * (p.sub_h + 1u) + threadIdx.x + 1u;
+    const uint32_t y = blockIdx.y * (p.sub_v + 1u) + threadIdx.y + 1u;
+    const uint32_t n = x + y * p.Hx;
+    const uint32_t code = p.code[n];
+    const bool fluid = code & LBM_CODE_FLUID;
+    C f[9];
+    if (fluid)
+        pull<L, S>(lat, n, code, p.Hx, p.N, f);
+    __syncthreads();
+    if (fluid)
+        finish_node<L, S>(p, lat, n, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
+}
+
+// nsol: one lattice, swap scheme (nsol_kernels.hpp:171-334).  Work-groups of
+// (sub_h+2) x (sub_v+1) overlap by one buffer column.  Every node hands its
+// populations 5..8 to the LDS slots of the neighbours that pull them and
+// reads populations 0..3 straight from the upstream nodes in HBM.  One
+// barrier separates all reads from the in-place writes (the reference uses
+// two LDS slots and three barriers; four slots need one).
+template <int L, int S>
+__global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    extern __shared__ unsigned char lds_raw[];
+    C *lds = reinterpret_cast<C *>(lds_raw);
+    T *lat = static_cast<T *>(p.dst);
+
+    const uint32_t P = p.sub_h + 4u;
+    const uint32_t plane = (p.sub_v + 2u) * P;
+    C *s5 = lds, *s6 = lds + plane, *s7 = lds + 2u * plane, *s8 = lds + 3u * plane;
+
+    const uint32_t lx = threadIdx.x + 1u, ly = threadIdx.y;
+    const uint32_t x = blockIdx.x * (p.sub_h + 1u) + threadIdx.x;
+    const uint32_t y = blockIdx.y * (p.sub_v + 1u) + threadIdx.y;
+    const uint32_t n = x + y * p.Hx;
+    const uint32_t N = p.N, Hx = p.Hx;
+    const uint32_t code = p.code[n];
+    const bool fluid = code & LBM_CODE_FLUID;
+
+    C f[9];
+    if (!(code & LBM_CODE_SOLID))
+    {
+        const C o5 = Storage<S>::load(lat, at<L>(n, 5, N), 5);
+        const C o6 = Storage<S>::load(lat, at<L>(n, 6, N), 6);
+        const C o7 = Storage<S>::load(lat, at<L>(n, 7, N), 7);
+        const C o8 = Storage<S>::load(lat, at<L>(n, 8, N), 8);
+        s5[ly * P + lx + 1u] = o5;
+        s6[(ly + 1u) * P + lx - 1u] = o6;
+        s7[(ly + 1u) * P + lx] = o7;
+        s8[(ly + 1u) * P + lx + 1u] = o8;
+        if (fluid)
+        {
+            f[4] = Storage<S>::load(lat, at<L>(n, 4, N), 4);
+            f[3] = (code & (1u << 3)) ? o5 : Storage<S>::load(lat, at<L>(n + 1u, 3, N), 3);
+            f[2] = (code & (1u << 2)) ? o6 : Storage<S>::load(lat, at<L>(n - 1u + Hx, 2, N), 2);
+            f[1] = (code & (1u << 1)) ? o7 : Storage<S>::load(lat, at<L>(n + Hx, 1, N), 1);
+            f[0] = (code & (1u << 0)) ? o8 : Storage<S>::load(lat, at<L>(n + 1u + Hx, 0, N), 0);
+            if (code & (1u << 5))
+                f[5] = Storage<S>::load(lat, at<L>(n, 3, N), 3);
+            if (code & (1u << 6))
+                f[6] = Storage<S>::load(lat, at<L>(n, 2, N), 2);
+            if (code & (1u << 7))
+                f[7] = Storage<S>::load(lat, at<L>(n, 1, N), 1);
+            if (code & (1u << 8))
+                f[8] = Storage<S>::load(lat, at<L>(n, 0, N), 0);
+        }
+    }
+    __syncthreads();
+    if (fluid)
+    {
+        const uint32_t own = ly * P + lx;
+        if (!(code & (1u << 5)))
+            f[5] = s5[own];
+        if (!(code & (1u << 6)))
+            f[6] = s6[own];
+        if (!(code & (1u << 7)))
+            f[7] = s7[own];
+        if (!(code & (1u << 8)))
+            f[8] = s8[own];
+        finish_node<L, S>(p, lat, n, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
+    }
+}
+
+// Buffer rows: directions 0..2 from the row above, 6..8 from the row below
+// (general/buffered.hpp:74-93).  Covers the full expanded width, ghost
+// columns included.
+template <int L, class T>
+__global__ void k_refresh_rows(T *lat, uint32_t Hx, uint32_t N, uint32_t sub_v)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= Hx)
+        return;
+    const uint32_t yb = (blockIdx.y + 1u) * (sub_v + 1u);
+    const uint32_t n = x + yb * Hx;
+#pragma unroll
+    for (uint32_t d = 0; d < 3; ++d)
+        lat[at<L>(n, d, N)] = lat[at<L>(n + Hx, d, N)];
+#pragma unroll
+    for (uint32_t d = 6; d < 9; ++d)
+        lat[at<L>(n, d, N)] = lat[at<L>(n - Hx, d, N)];
+}
+
+// Buffer columns: directions 2, 5, 8 from the left, 0, 3, 6 from the right
+// (general/buffered.hpp:133-152).  Runs after the rows so that the corner
+// nodes carry the diagonal neighbours' values.
+template <int L, class T>
+__global__ void k_refresh_cols(T *lat, uint32_t Hx, uint32_t Vx, uint32_t N, uint32_t sub_h)
+{
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= Vx)
+        return;
+    const uint32_t xb = (blockIdx.y + 1u) * (sub_h + 1u);
+    const uint32_t n = xb + y * Hx;
+#pragma unroll
+    for (uint32_t d = 2; d < 9; d += 3)
+        lat[at<L>(n, d, N)] = lat[at<L>(n - 1u, d, N)];
+#pragma unroll
+    for (uint32_t d = 0; d < 9; d += 3)
+        lat[at<L>(n, d, N)] = lat[at<L>(n + 1u, d, N)];
+}
+
+// ---------------------------------------------------------------------------
+// set-up kernels
+// ---------------------------------------------------------------------------
+template <int L, int S>
+__global__ void k_init_rest(typename Storage<S>::T *lat, uint32_t N)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+        lat[at<L>(n, uint32_t(d), N)] = Storage<S>::encode(weight<double>(d), d);
+}
+
+struct InoutValues
+{
+    double v[18];
+};
+
+// Inlet ghost column x = 0 and outlet ghost column x = H-1, rows 1..V-2, at
+// their equilibria (domain_initialization.hpp:382-398).
+template <int L, int S>
+__global__ void k_init_inout(typename Storage<S>::T *lat, uint32_t H, uint32_t V, uint32_t Hx, uint32_t N,
+                             uint32_t sub_h, uint32_t sub_v, int buffered, InoutValues vals)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t side = blockIdx.y;
+    if (j >= V - 2u)
+        return;
+    const uint32_t x = 1u + side * (H - 3u);
+    const uint32_t y = 1u + j;
+    uint32_t node = buffered ? (x + (x - 1u) / sub_h) + (y + (y - 1u) / sub_v) * Hx : x + y * Hx;
+    node = side ? node + 1u : node - 1u;
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+        lat[at<L>(node, uint32_t(d), N)] = Storage<S>::encode(vals.v[side * 9 + d], d);
+}
+
+struct CodeParams
+{
+    uint32_t H, V, Hx, N, sub_h, sub_v;
+    int buffered;
+    int outlet_active;
+};
+
+__device__ __forceinline__ uint32_t map_axis(uint32_t u, uint32_t last, uint32_t sub, int buffered)
+{
+    // unexpanded coordinate -> expanded (BufferedNodeAccess, access.hpp:353-362);
+    // the high ghost sits right after the last real node
+    if (!buffered)
+        return u;
+    if (u == 0u)
+        return 0u;
+    if (u == last)
+        return (last - 1u) + (last - 2u) / sub + 1u;
+    return u + (u - 1u) / sub;
+}
+
+__global__ void k_build_codes(const int8_t *phase, uint16_t *code, CodeParams cp)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= cp.N)
+        return;
+    const int8_t ph = phase[n];
+    uint32_t c = 0;
+    if (ph == 1)
+        c = LBM_CODE_SOLID;
+    else if (ph == 0)
+    {
+        const uint32_t xe = n % cp.Hx, ye = n / cp.Hx;
+        const uint32_t x = cp.buffered ? xe - xe / (cp.sub_h + 1u) : xe;
+        const uint32_t y = cp.buffered ? ye - ye / (cp.sub_v + 1u) : ye;
+        c = LBM_CODE_FLUID;
+        for (int d = 0; d < 9; ++d)
+        {
+            if (d == 4)
+                continue;
+            const uint32_t sx = uint32_t(int32_t(x) - cx_of(d));
+            const uint32_t sy = uint32_t(int32_t(y) - cy_of(d));
+            const uint32_t m = map_axis(sx, cp.H - 1u, cp.sub_h, cp.buffered) +
+                               map_axis(sy, cp.V - 1u, cp.sub_v, cp.buffered) * cp.Hx;
+            if (phase[m] == 1)
+                c |= 1u << d;
+        }
+        if (cp.outlet_active && x == cp.H - 2u && y >= 2u && y + 3u <= cp.V)
+            c |= LBM_CODE_OUTLET;
+        if (x + 3u == cp.H && y >= 2u && y + 3u <= cp.V)
+            c |= LBM_CODE_UY_SOURCE;
+    }
+    code[n] = uint16_t(c);
+}
+
+__global__ void k_fill(double *p, size_t n, double v)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+template <int L>
+__device__ __forceinline__ int dir_of_flat(uint32_t i, uint32_t N)
+{
+    if constexpr (L == LBM_LAYOUT_COLLISION)
+        return int(i % 9u);
+    else if constexpr (L == LBM_LAYOUT_STREAM)
+        return int(i / N);
+    else
+        return int(3u * (i / (3u * N)) + (i % (3u * N)) % 3u);
+}
+
+template <int L, int S>
+__global__ void k_decode(const typename Storage<S>::T *lat, double *out, uint32_t N)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9u * N)
+        return;
+    out[i] = Storage<S>::decode(lat[i], dir_of_flat<L>(i, N));
+}
+
+template <int L, int S>
+__global__ void k_encode(typename Storage<S>::T *lat, const double *in, uint32_t N)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9u * N)
+        return;
+    lat[i] = Storage<S>::encode(in[i], dir_of_flat<L>(i, N));
+}
+
+// Equilibrium of a given (rho, j) field on every fluid node (warm start).
+template <int L, int S>
+__global__ void k_set_equilibrium(typename Storage<S>::T *lat, const uint16_t *code, const double *rho,
+                                  const double *jx, const double *jy, uint32_t N, uint32_t Hx, uint32_t H,
+                                  uint32_t sub_h, uint32_t sub_v, int buffered)
+{
+    using C = typename Storage<S>::C;
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N || !(code[n] & LBM_CODE_FLUID))
+        return;
+    const uint32_t xe = n % Hx, ye = n / Hx;
+    const uint32_t x = buffered ? xe - xe / (sub_h + 1u) : xe;
+    const uint32_t y = buffered ? ye - ye / (sub_v + 1u) : ye;
+    const uint32_t r = (x - 1u) + (y - 1u) * (H - 2u);
+    const C rh = C(rho[r]), ux = C(jx[r]), uy = C(jy[r]);
+    const C usq = ux * ux + uy * uy;
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+    {
+        const C cu = C(cx_of(d)) * ux + C(cy_of(d)) * uy;
+        const C feq = weight<C>(d) * (rh + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
+        Storage<S>::store(lat, at<L>(n, uint32_t(d), N), feq, d);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions (wave64 shuffles, two deterministic passes)
+// ---------------------------------------------------------------------------
+constexpr int RED_BLOCK = 256;
+constexpr int RED_GRID = 1024;
+constexpr int RED_K = 6;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int K>
+__device__ __forceinline__ void block_sum_store(double (&acc)[K], double *out)
+{
+    __shared__ double part[RED_BLOCK / 64][K];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+    {
+        const double s = wave_sum(acc[k]);
+        if (lane == 0)
+            part[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < K)
+    {
+        double s = 0.0;
+        for (int w = 0; w < RED_BLOCK / 64; ++w)
+            s += part[w][threadIdx.x];
+        out[blockIdx.x * K + threadIdx.x] = s;
+    }
+}
+
+// mode 0: sum rho, jx, jy, count, momentum-exchange force (x, y);
+// mode 1: residual sums |j - j_snap|^2, |j|^2 and snapshot update.
+template <int L, int S>
+__global__ __launch_bounds__(RED_BLOCK) void k_reduce(const typename Storage<S>::T *lat, const uint16_t *code,
+                                                      uint32_t N, double *partials, double *snap, int mode)
+{
+    double acc[RED_K] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t n = blockIdx.x * RED_BLOCK + threadIdx.x; n < N; n += RED_GRID * RED_BLOCK)
+    {
+        const uint32_t c = code[n];
+        if (!(c & LBM_CODE_FLUID))
+            continue;
+        double f[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            f[d] = Storage<S>::decode(lat[at<L>(n, uint32_t(d), N)], d);
+        double rho = 0, jx = 0, jy = 0;
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+        {
+            rho += f[d];
+            jx += f[d] * double(cx_of(d));
+            jy += f[d] * double(cy_of(d));
+        }
+        if (mode == 0)
+        {
+            acc[0] += rho;
+            acc[1] += jx;
+            acc[2] += jy;
+            acc[3] += 1.0;
+#pragma unroll
+            for (int d = 0; d < 9; ++d)
+            {
+                // link n -> n + c_d ends in a solid iff the pull of 8-d bounces
+                if (d != 4 && ((c >> (8 - d)) & 1u))
+                {
+                    acc[4] += 2.0 * f[d] * double(cx_of(d));
+                    acc[5] += 2.0 * f[d] * double(cy_of(d));
+                }
+            }
+        }
+        else
+        {
+            const double dx = jx - snap[2u * n], dy = jy - snap[2u * n + 1u];
+            acc[0] += dx * dx + dy * dy;
+            acc[1] += jx * jx + jy * jy;
+            snap[2u * n] = jx;
+            snap[2u * n + 1u] = jy;
+        }
+    }
+    block_sum_store<RED_K>(acc, partials);
+}
+
+__global__ void k_reduce_final(const double *partials, int blocks, double *out)
+{
+    const int k = threadIdx.x;
+    if (k >= RED_K)
+        return;
+    double s = 0.0;
+    for (int b = 0; b < blocks; ++b)
+        s += partials[b * RED_K + k];
+    out[k] = s;
+}
+
+__global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        b[i] = a[i];
+}
+
+// ---------------------------------------------------------------------------
+// dispatch over layout x storage
+// ---------------------------------------------------------------------------
+template <int L, int S>
+struct Impl
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+
+    static int step(lbm_ctx *c, bool macros)
+    {
+        StepParams p{};
+        p.src = c->lat[c->cur];
+        p.dst = c->two_lattice ? c->lat[1 - c->cur] : c->lat[c->cur];
+        p.code = c->code;
+        p.rho = c->res;
+        p.jx = c->res + c->D;
+        p.jy = c->res + 2 * size_t(c->D);
+        p.jabs = c->res + 3 * size_t(c->D);
+        const int parity = int(c->iteration & 1u);
+        p.uy_in = c->uycol + size_t(parity) * c->V;
+        p.uy_out = c->uycol + size_t(1 - parity) * c->V;
+        p.omega = 1 / c->desc.relaxation_time;
+        p.rho_out = c->desc.outlet_density;
+        p.H = c->H;
+        p.V = c->V;
+        p.Hx = c->Hx;
+        p.Vx = c->Vx;
+        p.N = c->N;
+        p.sub_h = c->sub_h;
+        p.sub_v = c->sub_v;
+        p.write_macros = macros ? 1u : 0u;
+        p.nan_protection = c->desc.nan_protection ? 1u : 0u;
+
+        switch (c->desc.algorithm)
+        {
+        case LBM_ALG_NPTL:
+        {
+            dim3 grid((c->Hx - 2u + NPTL_BLOCK_X - 1u) / NPTL_BLOCK_X, c->Vx - 2u);
+            hipLaunchKernelGGL((k_step_nptl<L, S>), grid, dim3(NPTL_BLOCK_X), 0, c->stream, p);
+            break;
+        }
+        case LBM_ALG_LPTL:
+        {
+            const uint32_t b = c->wg;
+            hipLaunchKernelGGL((k_step_lptl<L, S>), dim3((c->N + b - 1u) / b), dim3(b), 0, c->stream, p);
+            break;
+        }
+        case LBM_ALG_NPOL:
+            refresh(c);
+            hipLaunchKernelGGL((k_step_npol<L, S>), dim3(c->cnt_h, c->cnt_v), dim3(c->sub_h, c->sub_v), 0,
+                               c->stream, p);
+            break;
+        case LBM_ALG_NSOL:
+        {
+            refresh(c);
+            const size_t lds = 4u * size_t(c->sub_v + 2u) * (c->sub_h + 4u) * sizeof(C);
+            hipLaunchKernelGGL((k_step_nsol<L, S>), dim3(c->cnt_h, c->cnt_v), dim3(c->sub_h + 2u, c->sub_v + 1u),
+                               lds, c->stream, p);
+            break;
+        }
+        }
+        LBM_HIP_TRY(hipGetLastError());
+        if (c->two_lattice)
+            c->cur ^= 1;
+        c->iteration++;
+        return LBM_OK;
+    }
+
+    static void refresh(lbm_ctx *c)
+    {
+        T *lat = static_cast<T *>(c->lat[c->cur]);
+        if (c->cnt_v > 1)
+            hipLaunchKernelGGL((k_refresh_rows<L, T>), dim3((c->Hx + 255u) / 256u, c->cnt_v - 1u), dim3(256), 0,
+                               c->stream, lat, c->Hx, c->N, c->sub_v);
+        if (c->cnt_h > 1)
+            hipLaunchKernelGGL((k_refresh_cols<L, T>), dim3((c->Vx + 255u) / 256u, c->cnt_h - 1u), dim3(256), 0,
+                               c->stream, lat, c->Hx, c->Vx, c->N, c->sub_h);
+    }
+
+    static int init(lbm_ctx *c)
+    {
+        const uint32_t blocks = (c->N + 255u) / 256u;
+        T *l0 = static_cast<T *>(c->lat[0]);
+        hipLaunchKernelGGL((k_init_rest<L, S>), dim3(blocks), dim3(256), 0, c->stream, l0, c->N);
+        InoutValues v;
+        for (int d = 0; d < 9; ++d)
+        {
+            v.v[d] = c->feq_in[d];
+            v.v[9 + d] = c->feq_out[d];
+        }
+        if (c->V > 2u)
+            hipLaunchKernelGGL((k_init_inout<L, S>), dim3((c->V - 2u + 255u) / 256u, 2), dim3(256), 0, c->stream,
+                               l0, c->H, c->V, c->Hx, c->N, c->sub_h, c->sub_v, c->buffered ? 1 : 0, v);
+        LBM_HIP_TRY(hipGetLastError());
+        if (c->two_lattice)
+            LBM_HIP_TRY(hipMemcpyAsync(c->lat[1], c->lat[0], 9ull * c->N * sizeof(T), hipMemcpyDeviceToDevice,
+                                       c->stream));
+        return LBM_OK;
+    }
+
+    static int decode(lbm_ctx *c, double *dev_out)
+    {
+        const uint32_t total = 9u * c->N;
+        hipLaunchKernelGGL((k_decode<L, S>), dim3((total + 255u) / 256u), dim3(256), 0, c->stream,
+                           static_cast<const T *>(c->lat[c->cur]), dev_out, c->N);
+        LBM_HIP_TRY(hipGetLastError());
+        return LBM_OK;
+    }
+
+    static int encode(lbm_ctx *c, const double *dev_in)
+    {
+        const uint32_t total = 9u * c->N;
+        hipLaunchKernelGGL((k_encode<L, S>), dim3((total + 255u) / 256u), dim3(256), 0, c->stream,
+                           static_cast<T *>(c->lat[c->cur]), dev_in, c->N);
+        LBM_HIP_TRY(hipGetLastError());
+        return LBM_OK;
+    }
+
+    static int equilibrium(lbm_ctx *c, const double *rho, const double *jx, const double *jy)
+    {
+        hipLaunchKernelGGL((k_set_equilibrium<L, S>), dim3((c->N + 255u) / 256u), dim3(256), 0, c->stream,
+                           static_cast<T *>(c->lat[c->cur]), c->code, rho, jx, jy, c->N, c->Hx, c->H, c->sub_h,
+                           c->sub_v, c->buffered ? 1 : 0);
+        LBM_HIP_TRY(hipGetLastError());
+        return LBM_OK;
+    }
+
+    static int reduce(lbm_ctx *c, int mode, double *dev_out6)
+    {
+        hipLaunchKernelGGL((k_reduce<L, S>), dim3(RED_GRID), dim3(RED_BLOCK), 0, c->stream,
+                           static_cast<const T *>(c->lat[c->cur]), c->code, c->N, c->partials, c->snap, mode);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, c->partials, RED_GRID, dev_out6);
+        LBM_HIP_TRY(hipGetLastError());
+        return LBM_OK;
+    }
+};
+
+struct VTable
+{
+    int (*step)(lbm_ctx *, bool);
+    int (*init)(lbm_ctx *);
+    int (*decode)(lbm_ctx *, double *);
+    int (*encode)(lbm_ctx *, const double *);
+    int (*equilibrium)(lbm_ctx *, const double *, const double *, const double *);
+    int (*reduce)(lbm_ctx *, int, double *);
+};
+
+template <int L, int S>
+static VTable make_vtable()
+{
+    return VTable{&Impl<L, S>::step, &Impl<L, S>::init, &Impl<L, S>::decode, &Impl<L, S>::encode,
+                  &Impl<L, S>::equilibrium, &Impl<L, S>::reduce};
+}
+
+template <int L>
+static VTable vtable_for_storage(int s)
+{
+    switch (s)
+    {
+    case LBM_STORE_F64: return make_vtable<L, LBM_STORE_F64>();
+    case LBM_STORE_F32: return make_vtable<L, LBM_STORE_F32>();
+    case LBM_STORE_F16_SHIFTED: return make_vtable<L, LBM_STORE_F16_SHIFTED>();
+    default: return make_vtable<L, LBM_STORE_BF16_SHIFTED>();
+    }
+}
+
+static VTable vtable(const lbm_ctx *c)
+{
+    switch (c->desc.layout)
+    {
+    case LBM_LAYOUT_COLLISION: return vtable_for_storage<LBM_LAYOUT_COLLISION>(c->desc.storage);
+    case LBM_LAYOUT_STREAM: return vtable_for_storage<LBM_LAYOUT_STREAM>(c->desc.storage);
+    default: return vtable_for_storage<LBM_LAYOUT_BUNDLE>(c->desc.storage);
+    }
+}
+
+static size_t storage_bytes(int s)
+{
+    switch (s)
+    {
+    case LBM_STORE_F64: return 8;
+    case LBM_STORE_F32: return 4;
+    default: return 2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *lbm_last_error(void) { return g_last_error.c_str(); }
+
+int lbm_device_max_work_group_size(uint32_t *out)
+{
+    if (!out)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_device_max_work_group_size: null output");
+    int dev = 0;
+    LBM_HIP_TRY(hipGetDevice(&dev));
+    int v = 0;
+    LBM_HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxThreadsPerBlock, dev));
+    *out = uint32_t(v);
+    return LBM_OK;
+}
+
+static int validate_desc(const lbm_desc *d)
+{
+    if (d->algorithm < LBM_ALG_NPTL || d->algorithm > LBM_ALG_NSOL)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "unknown algorithm");
+    if (d->layout < LBM_LAYOUT_COLLISION || d->layout > LBM_LAYOUT_BUNDLE)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "unknown data layout");
+    if (d->storage < LBM_STORE_F64 || d->storage > LBM_STORE_BF16_SHIFTED)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "unknown storage type");
+    if (d->horizontal_nodes < 4 || d->vertical_nodes < 4)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "horizontal_nodes and vertical_nodes must be >= 4");
+    if (!(d->relaxation_time > 0.0))
+        return fail(LBM_ERR_INVALID_ARGUMENT, "relaxation_time must be > 0");
+    if (d->work_group_size < 1 || d->work_group_size > 1024)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "work_group_size must be in [1, 1024]");
+    const uint64_t Hx = d->expanded_horizontal_nodes, Vx = d->expanded_vertical_nodes;
+    if (d->algorithm == LBM_ALG_LPTL && (Hx != d->horizontal_nodes || Vx != d->vertical_nodes))
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lptl runs on the unexpanded lattice");
+    if (Hx < d->horizontal_nodes || Vx < d->vertical_nodes)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "expanded extents smaller than the lattice");
+    if (9ull * Hx * Vx >= (1ull << 32))
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lattice too large for 32-bit population indices");
+    const uint64_t sh = d->subdomain_horizontal_nodes, sv = d->subdomain_vertical_nodes;
+    const uint64_t ch = d->subdomain_count_horizontal, cv = d->subdomain_count_vertical;
+    if (d->algorithm == LBM_ALG_NPOL || d->algorithm == LBM_ALG_NSOL)
+    {
+        if (sh < 1 || sv < 1 || ch < 1 || cv < 1)
+            return fail(LBM_ERR_INVALID_ARGUMENT, "subdomain extents must be >= 1");
+        if (Hx != (sh + 1) * ch + 1 || Vx != (sv + 1) * cv + 1)
+            return fail(LBM_ERR_INVALID_ARGUMENT, "expanded extents do not match the buffered subdomain grid");
+        if (sh * ch < d->horizontal_nodes - 2 || sv * cv < d->vertical_nodes - 2)
+            return fail(LBM_ERR_INVALID_ARGUMENT, "subdomain grid does not cover the lattice");
+        const uint64_t threads = d->algorithm == LBM_ALG_NPOL ? sh * sv : (sh + 2) * (sv + 1);
+        if (threads > 1024)
+            return fail(LBM_ERR_INVALID_ARGUMENT, "subdomain work-group exceeds 1024 work-items");
+    }
+    if (d->algorithm == LBM_ALG_NPTL && (Hx < 3 || Vx < 3))
+        return fail(LBM_ERR_INVALID_ARGUMENT, "nptl extents too small");
+    return LBM_OK;
+}
+
+void lbm_destroy(lbm_ctx *c)
+{
+    if (!c)
+        return;
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->lat[0]);
+    (void)hipFree(c->lat[1]);
+    (void)hipFree(c->phase);
+    (void)hipFree(c->code);
+    (void)hipFree(c->res);
+    (void)hipFree(c->uycol);
+    (void)hipFree(c->snap);
+    (void)hipFree(c->partials);
+    (void)hipFree(c->red_out);
+    if (c->ev0)
+        (void)hipEventDestroy(c->ev0);
+    if (c->ev1)
+        (void)hipEventDestroy(c->ev1);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int lbm_create(const lbm_desc *desc, lbm_ctx **out)
+{
+    if (!desc || !out)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_create: null argument");
+    *out = nullptr;
+    if (int rc = validate_desc(desc))
+        return rc;
+    lbm_ctx *c = new lbm_ctx();
+    c->desc = *desc;
+    c->H = desc->horizontal_nodes;
+    c->V = desc->vertical_nodes;
+    c->Hx = desc->expanded_horizontal_nodes;
+    c->Vx = desc->expanded_vertical_nodes;
+    c->N = c->Hx * c->Vx;
+    c->sub_h = std::max(1u, desc->subdomain_horizontal_nodes);
+    c->sub_v = std::max(1u, desc->subdomain_vertical_nodes);
+    c->cnt_h = desc->subdomain_count_horizontal;
+    c->cnt_v = desc->subdomain_count_vertical;
+    c->wg = desc->work_group_size;
+    c->buffered = desc->algorithm == LBM_ALG_NPOL || desc->algorithm == LBM_ALG_NSOL;
+    c->two_lattice = !c->buffered;
+    // The buffered outlet addresses the right neighbour of node H-3
+    // (general/buffered.hpp:285-288); when a buffer column separates H-3 and
+    // H-2 that neighbour is the buffer and the outlet never reaches the ghost
+    // column, which then keeps its initial equilibrium.
+    c->outlet_active = !(c->buffered && (c->H - 3u) % c->sub_h == 0u);
+    c->elem = storage_bytes(desc->storage);
+    c->D = (c->H - 2u) * (c->V - 2u);
+    host_equilibrium(desc->inlet_velocity_x, desc->inlet_velocity_y, desc->inlet_density, c->feq_in);
+    host_equilibrium(desc->outlet_velocity_x, desc->outlet_velocity_y, desc->outlet_density, c->feq_out);
+
+    auto cleanup = [&](int rc) {
+        lbm_destroy(c);
+        return rc;
+    };
+#define CREATE_TRY(expr)                                                                                 \
+    do                                                                                                   \
+    {                                                                                                    \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return cleanup(fail(e_ == hipErrorOutOfMemory ? LBM_ERR_OUT_OF_MEMORY : LBM_ERR_DEVICE,       \
+                                std::string(#expr) + ": " + hipGetErrorString(e_)));                      \
+    } while (0)
+
+    CREATE_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CREATE_TRY(hipEventCreate(&c->ev0));
+    CREATE_TRY(hipEventCreate(&c->ev1));
+    const size_t lat_bytes = 9ull * c->N * c->elem;
+    CREATE_TRY(hipMalloc(&c->lat[0], lat_bytes));
+    if (c->two_lattice)
+        CREATE_TRY(hipMalloc(&c->lat[1], lat_bytes));
+    CREATE_TRY(hipMalloc(&c->phase, c->N));
+    CREATE_TRY(hipMalloc(&c->code, size_t(c->N) * sizeof(uint16_t)));
+    CREATE_TRY(hipMalloc(&c->res, 4ull * std::max(1u, c->D) * sizeof(double)));
+    CREATE_TRY(hipMalloc(&c->uycol, 2ull * c->V * sizeof(double)));
+    CREATE_TRY(hipMalloc(&c->partials, size_t(RED_GRID) * RED_K * sizeof(double)));
+    CREATE_TRY(hipMalloc(&c->red_out, RED_K * sizeof(double)));
+    // Data fills phase with -1 (simulation.cpp:424); Results fill density -1,
+    // momentum 0 (simulation.cpp:407-410).
+    CREATE_TRY(hipMemsetAsync(c->phase, 0xff, c->N, c->stream));
+    CREATE_TRY(hipMemsetAsync(c->code, 0, size_t(c->N) * sizeof(uint16_t), c->stream));
+    CREATE_TRY(hipMemsetAsync(c->res, 0, 4ull * std::max(1u, c->D) * sizeof(double), c->stream));
+    CREATE_TRY(hipMemsetAsync(c->uycol, 0, 2ull * c->V * sizeof(double), c->stream));
+    if (c->D)
+    {
+        hipLaunchKernelGGL(k_fill, dim3((c->D + 255u) / 256u), dim3(256), 0, c->stream, c->res, size_t(c->D), -1.0);
+        CREATE_TRY(hipGetLastError());
+    }
+    CREATE_TRY(hipStreamSynchronize(c->stream));
+#undef CREATE_TRY
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_set_phase(lbm_ctx *c, const int8_t *phase, size_t n)
+{
+    if (!c || !phase)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_phase: null argument");
+    if (n != c->N)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_phase: node_count must equal the expanded node count");
+    LBM_HIP_TRY(hipMemcpyAsync(c->phase, phase, n, hipMemcpyHostToDevice, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->phase_set = true;
+    c->initialized = false;
+    return LBM_OK;
+}
+
+int lbm_get_phase(lbm_ctx *c, int8_t *phase, size_t n)
+{
+    if (!c || !phase)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_phase: null argument");
+    if (n != c->N)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_phase: node_count must equal the expanded node count");
+    LBM_HIP_TRY(hipMemcpyAsync(phase, c->phase, n, hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_init_populations(lbm_ctx *c)
+{
+    if (!c)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_init_populations: null context");
+    if (!c->phase_set)
+        return fail(LBM_ERR_STATE, "lbm_init_populations: lbm_set_phase must come first");
+    c->cur = 0;
+    if (int rc = vtable(c).init(c))
+        return rc;
+    CodeParams cp{c->H, c->V, c->Hx, c->N, c->sub_h, c->sub_v, c->buffered ? 1 : 0, c->outlet_active ? 1 : 0};
+    hipLaunchKernelGGL(k_build_codes, dim3((c->N + 255u) / 256u), dim3(256), 0, c->stream, c->phase, c->code, cp);
+    LBM_HIP_TRY(hipGetLastError());
+    LBM_HIP_TRY(hipMemsetAsync(c->uycol, 0, 2ull * c->V * sizeof(double), c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->iteration = 0;
+    c->snap_valid = false;
+    c->initialized = true;
+    return LBM_OK;
+}
+
+int lbm_get_node_codes(lbm_ctx *c, uint16_t *codes, size_t n)
+{
+    if (!c || !codes)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_node_codes: null argument");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_get_node_codes: call lbm_init_populations first");
+    if (n != c->N)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_node_codes: node_count must equal the expanded node count");
+    LBM_HIP_TRY(hipMemcpyAsync(codes, c->code, n * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_run(lbm_ctx *c, uint32_t steps, uint32_t frame_interval)
+{
+    if (!c)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_run: null context");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_run: call lbm_init_populations first");
+    const VTable vt = vtable(c);
+    LBM_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (uint32_t i = 0; i < steps; ++i)
+    {
+        const uint64_t it = c->iteration;
+        const bool frame = frame_interval ? (it % frame_interval == frame_interval - 1u) : false;
+        const bool macros = frame || i + 1u == steps;
+        if (int rc = vt.step(c, macros))
+            return rc;
+    }
+    LBM_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    LBM_HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    LBM_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return LBM_OK;
+}
+
+int lbm_get_iteration(lbm_ctx *c, uint64_t *it)
+{
+    if (!c || !it)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_iteration: null argument");
+    *it = c->iteration;
+    return LBM_OK;
+}
+
+int lbm_last_run_ms(lbm_ctx *c, double *ms)
+{
+    if (!c || !ms)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_last_run_ms: null argument");
+    *ms = c->last_ms;
+    return LBM_OK;
+}
+
+int lbm_get_macros(lbm_ctx *c, double *rho, double *jx, double *jy, double *jabs, size_t count)
+{
+    if (!c)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_macros: null context");
+    if (count != c->D)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_macros: count must equal (H-2)*(V-2)");
+    double *dst[4] = {rho, jx, jy, jabs};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k] && count)
+            LBM_HIP_TRY(hipMemcpyAsync(dst[k], c->res + size_t(k) * c->D, count * sizeof(double),
+                                       hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_get_populations(lbm_ctx *c, double *values, size_t count)
+{
+    if (!c || !values)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_populations: null argument");
+    if (count != 9ull * c->N)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_populations: count must be 9 * expanded node count");
+    double *tmp = nullptr;
+    LBM_HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
+    int rc = vtable(c).decode(c, tmp);
+    if (!rc)
+    {
+        hipError_t e = hipMemcpyAsync(values, tmp, count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess)
+            rc = fail(LBM_ERR_DEVICE, std::string("lbm_get_populations: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(tmp);
+    return rc;
+}
+
+int lbm_set_populations(lbm_ctx *c, const double *values, size_t count)
+{
+    if (!c || !values)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_populations: null argument");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_set_populations: call lbm_init_populations first");
+    if (count != 9ull * c->N)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_populations: count must be 9 * expanded node count");
+    double *tmp = nullptr;
+    LBM_HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, values, count * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    int rc = e == hipSuccess ? vtable(c).encode(c, tmp)
+                             : fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
+    if (!rc && c->two_lattice)
+    {
+        e = hipMemcpyAsync(c->lat[1 - c->cur], c->lat[c->cur], 9ull * c->N * c->elem, hipMemcpyDeviceToDevice,
+                           c->stream);
+        if (e != hipSuccess)
+            rc = fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (!rc && e != hipSuccess)
+        rc = fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
+    (void)hipFree(tmp);
+    return rc;
+}
+
+int lbm_set_equilibrium_field(lbm_ctx *c, const double *rho, const double *jx, const double *jy, size_t count)
+{
+    if (!c || !rho || !jx || !jy)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_equilibrium_field: null argument");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_set_equilibrium_field: call lbm_init_populations first");
+    if (count != c->D)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_equilibrium_field: count must equal (H-2)*(V-2)");
+    double *tmp = nullptr;
+    LBM_HIP_TRY(hipMalloc(&tmp, 3 * count * sizeof(double)));
+    hipError_t e = hipSuccess;
+    const double *src[3] = {rho, jx, jy};
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+        e = hipMemcpyAsync(tmp + k * count, src[k], count * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    int rc = e == hipSuccess ? vtable(c).equilibrium(c, tmp, tmp + count, tmp + 2 * count)
+                             : fail(LBM_ERR_DEVICE, std::string("lbm_set_equilibrium_field: ") + hipGetErrorString(e));
+    e = hipStreamSynchronize(c->stream);
+    if (!rc && e != hipSuccess)
+        rc = fail(LBM_ERR_DEVICE, std::string("lbm_set_equilibrium_field: ") + hipGetErrorString(e));
+    (void)hipFree(tmp);
+    c->snap_valid = false;
+    return rc;
+}
+
+static int reduce_common(lbm_ctx *c, int mode, double host_out[RED_K])
+{
+    double *out_tmp = c->red_out;
+    int rc = vtable(c).reduce(c, mode, out_tmp);
+    if (!rc)
+    {
+        hipError_t e = hipMemcpyAsync(host_out, out_tmp, RED_K * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess)
+            rc = fail(LBM_ERR_DEVICE, std::string("reduction: ") + hipGetErrorString(e));
+    }
+    return rc;
+}
+
+int lbm_reduce(lbm_ctx *c, double out[6])
+{
+    if (!c || !out)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_reduce: null argument");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_reduce: call lbm_init_populations first");
+    return reduce_common(c, 0, out);
+}
+
+int lbm_residual(lbm_ctx *c, double *residual)
+{
+    if (!c || !residual)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_residual: null argument");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_residual: call lbm_init_populations first");
+    if (!c->snap)
+        LBM_HIP_TRY(hipMalloc(&c->snap, 2ull * c->N * sizeof(double)));
+    if (!c->snap_valid)
+    {
+        LBM_HIP_TRY(hipMemsetAsync(c->snap, 0, 2ull * c->N * sizeof(double), c->stream));
+        c->snap_valid = true;
+    }
+    double out[RED_K];
+    if (int rc = reduce_common(c, 1, out))
+        return rc;
+    *residual = out[1] > 0.0 ? std::sqrt(out[0] / out[1]) : (out[0] > 0.0 ? 1.0 : 0.0);
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_ctx *c)
+{
+    if (!c)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_sync: null context");
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_measure_copy_bandwidth(size_t bytes, int repeats, double *gbps)
+{
+    if (!gbps || bytes < 16 || repeats < 1)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_measure_copy_bandwidth: bad argument");
+    bytes &= ~size_t(15);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LBM_OK;
+    auto check = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == LBM_OK)
+            rc = fail(LBM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        return e == hipSuccess;
+    };
+    if (check(hipMalloc(&a, bytes), "hipMalloc") && check(hipMalloc(&b, bytes), "hipMalloc") &&
+        check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate") &&
+        check(hipEventCreate(&e0), "hipEventCreate") && check(hipEventCreate(&e1), "hipEventCreate") &&
+        check(hipMemsetAsync(a, 0, bytes, s), "hipMemsetAsync"))
+    {
+        const size_t n = bytes / 16;
+        double best = 0.0;
+        for (int r = 0; r <= repeats && rc == LBM_OK; ++r)
+        {
+            check(hipEventRecord(e0, s), "hipEventRecord");
+            hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, s, static_cast<const uint4 *>(a),
+                               static_cast<uint4 *>(b), n);
+            check(hipGetLastError(), "k_copy16");
+            check(hipEventRecord(e1, s), "hipEventRecord");
+            check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (r > 0 && ms > 0.f)
+                best = std::max(best, 2.0 * double(bytes) / (double(ms) * 1e6));
+        }
+        *gbps = best;
+    }
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    if (s)
+        (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,266 @@
+// HipAlgorithmHandler: the reference's handler lifecycle on the GPU library.
+#include "lbm/execution/hip_algorithm_handler.hpp"
+
+#include <algorithm>
+#include <string>
+
+#include "lbm/core/domain_initialization.hpp"
+#include "lbm/exceptions/exceptions.hpp"
+#include "lbm/file_interaction/file_interaction.hpp"
+#include "lbm/portlbm_factory.hpp"
+#include "lbm_hip.h"
+
+namespace lbm
+{
+namespace execution
+{
+
+namespace
+{
+// Steps per lbm_run call when no frame step comes earlier: long enough to
+// keep launch and synchronisation cost negligible, short enough for pause()
+// to take effect quickly.
+constexpr unsigned int max_batch = 512;
+
+void check(int rc, const char *what)
+{
+    if (rc != LBM_OK)
+    {
+        const std::string msg = std::string(what) + ": " + lbm_last_error();
+        if (rc == LBM_ERR_INVALID_ARGUMENT)
+            throw exceptions::json::PropertyArgumentException(msg);
+        throw exceptions::Exception(msg);
+    }
+}
+
+int algorithm_id(const std::string &a)
+{
+    if (a == "nptl")
+        return LBM_ALG_NPTL;
+    if (a == "lptl")
+        return LBM_ALG_LPTL;
+    if (a == "npol")
+        return LBM_ALG_NPOL;
+    if (a == "nsol")
+        return LBM_ALG_NSOL;
+    throw exceptions::Exception("Unknown algorithm: " + a);
+}
+
+int layout_id(const std::string &l)
+{
+    if (l == "collision")
+        return LBM_LAYOUT_COLLISION;
+    if (l == "stream")
+        return LBM_LAYOUT_STREAM;
+    if (l == "bundle")
+        return LBM_LAYOUT_BUNDLE;
+    throw exceptions::Exception("Unknown data layout: " + l);
+}
+
+int storage_id(const std::string &s)
+{
+    if (s == "f64")
+        return LBM_STORE_F64;
+    if (s == "f32")
+        return LBM_STORE_F32;
+    if (s == "f16-shifted")
+        return LBM_STORE_F16_SHIFTED;
+    if (s == "bf16-shifted")
+        return LBM_STORE_BF16_SHIFTED;
+    throw exceptions::json::PropertyArgumentException("Unknown storage: " + s);
+}
+}  // namespace
+
+HipAlgorithmHandler::HipAlgorithmHandler(const std::string &settings_path)
+    : AlgorithmHandler(0), settings_path_(settings_path)
+{
+    initialize_algorithm();
+}
+
+HipAlgorithmHandler::HipAlgorithmHandler(core::Properties props)
+    : AlgorithmHandler(0), settings_path_(props.settings_path), explicit_props_(std::move(props))
+{
+    initialize_algorithm();
+}
+
+HipAlgorithmHandler::~HipAlgorithmHandler() { release(); }
+
+void HipAlgorithmHandler::release()
+{
+    if (control_)
+        control_->forbid_execution();
+    if (future_.valid())
+        future_.wait();
+    if (ctx_)
+        lbm_destroy(ctx_);
+    ctx_ = nullptr;
+    active_ = false;
+}
+
+void HipAlgorithmHandler::initialize_algorithm()
+{
+    core::Properties props =
+        explicit_props_.has_value() ? *explicit_props_ : file_interaction::json_to_properties(settings_path_);
+    const int alg = algorithm_id(props.algorithm);
+    const int layout = layout_id(props.data_layout);
+    const int storage = storage_id(props.extensions.storage);
+
+    uint32_t device_max = 0;
+    check(lbm_device_max_work_group_size(&device_max), "device query");
+    processing_element_constraint = device_max;
+    // simulation.cpp:487-499
+    if (props.work_group_size > device_max)
+        throw exceptions::json::PropertyArgumentException(
+            "work_group_size " + std::to_string(props.work_group_size) + " exceeds the device maximum of " +
+            std::to_string(device_max) + "; reduce it to " + std::to_string(device_max) + " and reconstruct.");
+
+    properties_ = std::make_unique<core::Properties>(props);
+    control_ = std::make_unique<core::Control>(properties_->time_steps);
+    domain_ = std::make_unique<core::Domain>(*properties_);
+
+    // Results start at density -1 and zero momentum (simulation.cpp:386-417).
+    const std::size_t D = properties_->domain_node_count;
+    densities_.assign(D, -1.0);
+    x_velocities_.assign(D, 0.0);
+    y_velocities_.assign(D, 0.0);
+    absolute_velocities_.assign(D, 0.0);
+
+    phase_ = core::domain_initialization::build_phase(*properties_, *domain_);
+
+    lbm_desc desc{};
+    desc.algorithm = alg;
+    desc.layout = layout;
+    desc.storage = storage;
+    desc.nan_protection = properties_->extensions.nan_protection ? 1 : 0;
+    desc.horizontal_nodes = properties_->horizontal_nodes;
+    desc.vertical_nodes = properties_->vertical_nodes;
+    desc.expanded_horizontal_nodes = domain_->horizontal_nodes;
+    desc.expanded_vertical_nodes = domain_->vertical_nodes;
+    desc.subdomain_horizontal_nodes = domain_->subdomain_horizontal_nodes;
+    desc.subdomain_vertical_nodes = domain_->subdomain_vertical_nodes;
+    desc.subdomain_count_horizontal = domain_->subdomain_count_horizontal;
+    desc.subdomain_count_vertical = domain_->subdomain_count_vertical;
+    desc.work_group_size = properties_->work_group_size;
+    desc.relaxation_time = properties_->relaxation_time;
+    desc.inlet_density = properties_->inlet_density;
+    desc.inlet_velocity_x = properties_->inlet_velocity_x;
+    desc.inlet_velocity_y = properties_->inlet_velocity_y;
+    desc.outlet_density = properties_->outlet_density;
+    desc.outlet_velocity_x = properties_->outlet_velocity_x;
+    desc.outlet_velocity_y = properties_->outlet_velocity_y;
+
+    check(lbm_create(&desc, &ctx_), "lbm_create");
+    check(lbm_set_phase(ctx_, phase_.data(), phase_.size()), "lbm_set_phase");
+    check(lbm_init_populations(ctx_), "lbm_init_populations");
+    // a ready future, so that block_until_finished() before start() returns
+    // at once, as in the reference (sycl_algorithm.hpp:50-53)
+    future_ = std::async(std::launch::deferred, [] {});
+    active_ = false;
+}
+
+void HipAlgorithmHandler::initialize()
+{
+    release();
+    initialize_algorithm();
+}
+
+void HipAlgorithmHandler::copy_macroscopic_observables_to_cpu()
+{
+    check(lbm_get_macros(ctx_, densities_.data(), x_velocities_.data(), y_velocities_.data(),
+                         absolute_velocities_.data(), densities_.size()),
+          "lbm_get_macros");
+}
+
+// The reference's execute() loop (nptl.hpp:116-141) in batches: each batch
+// ends at the next frame step, the last step, or after max_batch steps.
+void HipAlgorithmHandler::execute()
+{
+    const unsigned int fui = properties_->frame_update_interval;
+    while (control_->is_execution_allowed())
+    {
+        const unsigned int it = control_->get_current_iteration();
+        const unsigned int remaining = control_->get_max_iterations() - it;
+        const unsigned int to_frame = fui ? fui - (it % fui) : remaining;
+        const unsigned int batch = std::min({remaining, to_frame, max_batch});
+        control_->reset_timer();
+        check(lbm_run(ctx_, batch, fui), "lbm_run");
+        control_->finalize_iterations(batch, control_->elapsed_ms());
+        const unsigned int last = it + batch - 1;
+        if (fui && last % fui == fui - 1)
+            copy_macroscopic_observables_to_cpu();
+    }
+    if (control_->is_finished())
+        copy_macroscopic_observables_to_cpu();
+}
+
+void HipAlgorithmHandler::start()
+{
+    if (!active_)
+    {
+        control_->allow_execution();
+        future_ = std::async(std::launch::async, [this] { execute(); });
+        active_ = true;
+    }
+}
+
+void HipAlgorithmHandler::pause()
+{
+    control_->forbid_execution();
+    if (future_.valid())
+        future_.wait();
+    active_ = false;
+}
+
+bool HipAlgorithmHandler::is_finished() const { return control_->is_finished(); }
+
+void HipAlgorithmHandler::block_until_finished()
+{
+    try
+    {
+        future_.get();
+    }
+    catch (const std::exception &e)
+    {
+        active_ = false;
+        throw exceptions::algorithm::WaitException(
+            std::string("A thread accessed a blocking statement causing it to wait for an algorithm that has not "
+                        "been launched. (") +
+            e.what() + ")");
+    }
+    check(lbm_sync(ctx_), "lbm_sync");
+    active_ = false;
+}
+
+std::vector<double> HipAlgorithmHandler::get_populations() const
+{
+    std::vector<double> values(9ull * domain_->total_node_count);
+    check(lbm_get_populations(ctx_, values.data(), values.size()), "lbm_get_populations");
+    return values;
+}
+
+}  // namespace execution
+
+std::unique_ptr<execution::AlgorithmHandler> create_handler(const core::Properties &props)
+{
+    return std::make_unique<execution::HipAlgorithmHandler>(props);
+}
+
+std::unique_ptr<execution::AlgorithmHandler> create_handler(const std::string &settings_path)
+{
+    return std::make_unique<execution::HipAlgorithmHandler>(settings_path);
+}
+
+namespace exceptions
+{
+std::string Exception::to_string() const
+{
+    return std::string(exception_name_) +
+           " thrown:\n"
+           "-------------------------------------------------------------------------------\n"
+           "What:\t\t" +
+           what() + "\nFile:\t\t" + source_location_.file_name() + "\nFunction:\t" +
+           source_location_.function_name() + "\nLine:\t\t" + std::to_string(source_location_.line()) + "\n";
+}
+}  // namespace exceptions
+
+}  // namespace lbm

@@ -1,0 +1,65 @@
+"""Parity of the GPU library (through the C ABI, include/lbm_hip.h) with the
+CPU oracle on identical inputs.
+
+Bar (plain FP64 path): node-type table and node-code (bounce-back
+neighbour) table bit-exact; fluid populations and the four result arrays
+within 1e-12 relative after N steps.
+"""
+import numpy as np
+import pytest
+
+import portlbm_amd as pa
+from tests.parity import FP64_RTOL, compare
+
+pytestmark = pytest.mark.gpu
+
+ALGS = ["nptl", "lptl", "npol", "nsol"]
+LAYOUTS = ["collision", "stream", "bundle"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("alg", ALGS)
+def test_hagen_poiseuille_256x64_1000_steps(alg, layout):
+    """256 x 64 Hagen-Poiseuille, tau 0.6, inlet 1.2 / outlet 1.0, 1000 steps."""
+    doc = pa.settings(alg, layout, inner_h=256, inner_v=64, tau=0.6, rho_in=1.2, rho_out=1.0,
+                      time_steps=1000, work_group_size=256)
+    r = compare(doc, 1000)
+    print(alg, layout, r, flush=True)
+    assert r["phase_equal"] and r["codes_equal"]
+    assert r["fluid_nodes"] == 256 * 62
+    assert r["pop_err"] <= FP64_RTOL
+    assert r["field_err"] <= FP64_RTOL
+
+
+# Ragged shapes: padding in nptl, partial tiles in npol/nsol, odd and
+# non-power-of-two work-groups, and the buffered-outlet quirk shape
+# (H-3 a multiple of sub_h: the outlet never reaches the ghost column).
+RAGGED = [
+    ("nptl", 250, 70, 64), ("nptl", 37, 21, 7), ("nptl", 100, 30, 12), ("nptl", 33, 10, 1024),
+    ("lptl", 37, 21, 7), ("lptl", 100, 30, 96),
+    ("npol", 250, 70, 64), ("npol", 37, 21, 7), ("npol", 100, 30, 12), ("npol", 33, 18, 16),
+    ("nsol", 250, 70, 64), ("nsol", 37, 21, 7), ("nsol", 100, 30, 12), ("nsol", 29, 16, 64),
+    ("nsol", 20, 12, 6), ("npol", 2, 2, 4), ("nptl", 2, 2, 4),
+]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("alg,w,h,wg", RAGGED)
+def test_ragged_shapes(alg, w, h, wg, layout):
+    doc = pa.settings(alg, layout, inner_h=w, inner_v=h, work_group_size=wg, time_steps=300,
+                      scenario="circle", tau=0.7, rho_in=1.05)
+    r = compare(doc, 300)
+    print(alg, w, h, wg, layout, r, flush=True)
+    assert r["ok"], r
+
+
+@pytest.mark.parametrize("scenario", ["walls", "circle", "square", "plate", "skyscraper", "wing",
+                                      "porous-circles", "cylinder"])
+@pytest.mark.parametrize("alg", ALGS)
+def test_scenarios(alg, scenario):
+    doc = pa.settings(alg, "stream", inner_h=300, inner_v=90, work_group_size=64, time_steps=400,
+                      scenario=scenario, tau=0.65, rho_in=1.03,
+                      porosity=0.75, obstacleMargin=10, cylinderX=80, cylinderRadius=15.0)
+    r = compare(doc, 400)
+    print(alg, scenario, r, flush=True)
+    assert r["ok"], r
