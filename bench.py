@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the D2Q9 BGK time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--algorithm nptl]
+                    [--layout stream] [--storage f64] [--sweep OUT.json]
+
+Workload: the 8192 x 2048 porous channel (seeded non-overlapping discs,
+radii U[4, 16), porosity 0.7), tau 0.6, inlet density 1.05, outlet 1.0.
+A step is one fused time step of the whole lattice with the inputs resident
+in HBM.  The unit of work is a fluid-node update; the line reports MFLUPS
+(10^6 fluid-node updates per second), the kernel's achieved HBM bandwidth
+against the MI355X peak (algorithmic bytes: 2 x 9 x sizeof(storage) per
+fluid node), and the CPU oracle timed on the host cores as a baseline.
+
+Prints one JSON line.  --sweep additionally times every algorithm x layout x
+storage combination and writes them to a JSON file.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+import portlbm_amd as pa  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+INNER_H, INNER_V = 8192, 2048
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+
+
+def workload(algorithm: str, layout: str, storage: str, steps: int, scenario: str = "porous-circles") -> dict:
+    wg = 1024 if algorithm in ("npol", "nsol") else 256
+    return pa.settings(algorithm, layout, inner_h=INNER_H, inner_v=INNER_V, scenario=scenario,
+                       work_group_size=wg, time_steps=steps, tau=0.6, rho_in=1.05, rho_out=1.0,
+                       storage=storage, seed=20261202, porosity=0.7, radiusMin=4.0, radiusMax=16.0,
+                       obstacleMargin=32)
+
+
+def time_gpu(doc: dict, steps: int, warmup: int) -> dict:
+    lat = pa.Lattice.from_settings(doc)
+    try:
+        fluid = int((lat.node_codes() & pa.CODE_FLUID).astype(bool).sum())
+        if warmup:
+            lat.run(warmup, 0)
+        lat.sync()
+        t0 = time.perf_counter()
+        dev_ms = lat.run(steps, 0)  # HIP events on the lattice's stream around the step launches
+        lat.sync()
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        red = lat.reduce()
+        return {"fluid": fluid, "dev_ms": dev_ms, "wall_ms": wall_ms, "mass": red["mass"]}
+    finally:
+        lat.close()
+
+
+def storage_bytes(storage: str) -> int:
+    return pa.STORAGE_BYTES[storage]
+
+
+def cpu_baseline(algorithm: str, layout: str, budget_s: float = 15.0) -> dict:
+    """The CPU oracle (the reference's pass structure: bounce-back emplacement,
+    outlet, stream-collide with macroscopic outputs every step) on the same
+    workload, OpenMP over rows on the host cores, for a bounded number of steps."""
+    from oracle import binding as ob
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    ob.set_threads(cores)
+    doc = workload(algorithm, layout, "f64", 1)
+    sim = ob.Simulation(doc)
+    try:
+        fluid = int((sim.phase() == 0).sum())
+        sim.run(1)  # warm-up
+        t0 = time.perf_counter()
+        sim.run(1)
+        one = time.perf_counter() - t0
+        steps = max(1, min(200, int(budget_s / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        sim.run(steps)
+        dt = time.perf_counter() - t0
+    finally:
+        sim.close()
+    return {"value": round(fluid * steps / dt / 1e6, 2), "unit": "MFLUPS", "cores": cores, "kind": "port",
+            "sample": f"oracle {algorithm}/{layout} FP64 on the same 8192x2048 porous channel, "
+                      f"{steps} steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def measured_traffic(algorithm: str, layout: str, storage: str) -> float | None:
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if it
+    covers this kernel and workload."""
+    try:
+        data = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    rec = data.get(f"{algorithm}/{layout}/{storage}/{INNER_H}x{INNER_V}")
+    return None if rec is None else float(rec["bytes_per_launch"])
+
+
+def sweep(out_path: str, steps: int, warmup: int) -> None:
+    rows = []
+    for storage in ("f64", "f32", "f16-shifted", "bf16-shifted"):
+        for algorithm in ("nptl", "lptl", "npol", "nsol"):
+            for layout in ("collision", "stream", "bundle"):
+                r = time_gpu(workload(algorithm, layout, storage, steps), steps, warmup)
+                mflups = r["fluid"] * steps / (r["dev_ms"] * 1e-3) / 1e6
+                gbps = mflups * 1e6 * 2 * 9 * storage_bytes(storage) / 1e9
+                row = {"algorithm": algorithm, "layout": layout, "storage": storage, "mflups": round(mflups, 1),
+                       "mlups_all": round(INNER_H * INNER_V * steps / (r["dev_ms"] * 1e-3) / 1e6, 1),
+                       "gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4),
+                       "ms_per_step": round(r["dev_ms"] / steps, 4)}
+                rows.append(row)
+                print(json.dumps(row), file=sys.stderr, flush=True)
+    Path(out_path).write_text(json.dumps({"workload": f"porous-circles {INNER_H}x{INNER_V}", "steps": steps,
+                                          "rows": rows}, indent=1))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--algorithm", default="nptl", choices=sorted(pa.ALGORITHMS))
+    ap.add_argument("--layout", default="stream", choices=sorted(pa.LAYOUTS))
+    ap.add_argument("--storage", default="f64", choices=sorted(pa.STORAGES))
+    ap.add_argument("--scenario", default="porous-circles",
+                    help="porous-circles (the benchmark workload) or any other scenario, for comparison")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep", metavar="OUT.json")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only).", file=sys.stderr)
+        return 2
+
+    if args.sweep:
+        sweep(args.sweep, min(args.steps, 500), min(args.warmup, 20))
+
+    doc = workload(args.algorithm, args.layout, args.storage, args.steps, args.scenario)
+    r = time_gpu(doc, args.steps, args.warmup)
+    secs = r["dev_ms"] * 1e-3
+    mflups = r["fluid"] * args.steps / secs / 1e6
+    bytes_per_node = 2 * 9 * storage_bytes(args.storage)
+    launch_ms = r["dev_ms"] / args.steps  # one kernel launch per step for nptl/lptl
+    achieved = r["fluid"] * bytes_per_node / (launch_ms * 1e-3) / 1e9
+    line = {
+        "metric": "MLUPS & % HBM roofline at 8192x2048; steps-to-steady warm-start vs plain",
+        "value": round(mflups, 1),
+        "unit": "MFLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["wall_ms"] / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32": "f32", "f16-shifted": "f16-storage/f32", "bf16-shifted":
+                  "bf16-storage/f32"}[args.storage],
+        "data": "synthetic",
+        "config": {"workload": (f"porous-circles channel {INNER_H}x{INNER_V}, discs r~U[4,16) porosity 0.7 "
+                                f"seed 20261202, tau 0.6, rho_in 1.05 / rho_out 1.0"
+                                if args.scenario == "porous-circles" else
+                                f"{args.scenario} channel {INNER_H}x{INNER_V}, tau 0.6, rho_in 1.05 / rho_out 1.0"),
+                   "algorithm": args.algorithm, "layout": args.layout, "storage": args.storage,
+                   "fluid_nodes": r["fluid"], "inner_nodes": INNER_H * INNER_V},
+        "mlups_all_nodes": round(INNER_H * INNER_V * args.steps / secs / 1e6, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic(args.algorithm, args.layout, args.storage)},
+        "kernel_ms": round(launch_ms, 5),
+        "bytes_per_fluid_node": bytes_per_node,
+        "mass": r["mass"],
+    }
+    try:
+        line["copy_bandwidth_gbps"] = round(pa.copy_bandwidth_gbps(1 << 30, 5), 1)
+    except pa.LbmError:
+        line["copy_bandwidth_gbps"] = None
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.algorithm, args.layout)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

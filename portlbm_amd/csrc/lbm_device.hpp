@@ -52,6 +52,7 @@ struct Storage<LBM_STORE_F64>
     using C = double;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
     static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
     static __device__ __forceinline__ T encode(double v, int) { return v; }
     static __device__ __forceinline__ double decode(T v, int) { return v; }
 };
@@ -63,6 +64,7 @@ struct Storage<LBM_STORE_F32>
     using C = float;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
     static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
     static __device__ __forceinline__ T encode(double v, int) { return float(v); }
     static __device__ __forceinline__ double decode(T v, int) { return double(v); }
 };
@@ -80,6 +82,10 @@ struct Storage<LBM_STORE_F16_SHIFTED>
     static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int d)
     {
         p[i] = _Float16(v - weight<float>(d));
+    }
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int d)
+    {
+        __builtin_nontemporal_store(_Float16(v - weight<float>(d)), p + i);
     }
     static __device__ __forceinline__ T encode(double v, int d) { return _Float16(float(v - weight<double>(d))); }
     static __device__ __forceinline__ double decode(T v, int d) { return double(float(v)) + weight<double>(d); }
@@ -110,16 +116,39 @@ struct Storage<LBM_STORE_BF16_SHIFTED>
     {
         p[i] = f32_to_bf16(v - weight<float>(d));
     }
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int d)
+    {
+        __builtin_nontemporal_store(f32_to_bf16(v - weight<float>(d)), p + i);
+    }
     static __device__ __forceinline__ T encode(double v, int d) { return f32_to_bf16(float(v - weight<double>(d))); }
     static __device__ __forceinline__ double decode(T v, int d) { return double(bf16_to_f32(v)) + weight<double>(d); }
 };
+
+// ---- physical placement of the lattice in HBM ----------------------------
+// The logical node numbering is the reference's, n = x + y * Hx on the
+// expanded lattice.  In HBM each row is padded to a pitch P that is a
+// multiple of 64 nodes and starts X0 = 63 nodes into the row, so that the
+// first interior node x = 1 of every row begins a 64-node (512 B in FP64)
+// aligned segment; node (x, y) lives at p = X0 + x + y * P.  Layout formulas
+// (access.hpp) are then applied to p with the per-direction stride S >= P*Vx,
+// rounded up so that every direction array starts on a 2 MiB boundary.
+struct Placement
+{
+    uint32_t P;   // row pitch (nodes)
+    uint32_t X0;  // offset of x = 0 within a row (nodes)
+    uint32_t S;   // stride between direction arrays (nodes)
+};
+
+constexpr uint32_t kRowOffset = 63;
+
+__device__ __forceinline__ uint32_t place(uint32_t x, uint32_t y, uint32_t P, uint32_t X0) { return X0 + x + y * P; }
 
 // ---- step parameters ------------------------------------------------------
 struct StepParams
 {
     const void *src;  // source lattice
     void *dst;        // destination lattice (== src for one-lattice schemes)
-    const uint16_t *code;
+    const uint16_t *code;  // node codes, physical placement
     double *rho;      // results, (H-2)*(V-2) each
     double *jx;
     double *jy;
@@ -130,7 +159,8 @@ struct StepParams
     double rho_out;
     uint32_t H, V;        // unexpanded extents incl. ghosts
     uint32_t Hx, Vx;      // expanded extents
-    uint32_t N;           // Hx * Vx
+    uint32_t N;           // Hx * Vx (logical node count)
+    uint32_t P, X0, S;    // physical placement
     uint32_t sub_h, sub_v;
     uint32_t write_macros;
     uint32_t nan_protection;
@@ -149,17 +179,18 @@ __device__ __forceinline__ C clamp_output(C v, uint32_t on)
 // ---- the update of one fluid node ------------------------------------------
 // Pull 9 populations.  Bit d of `code` set: the upstream node n - c_d is solid
 // and half-way bounce-back supplies this node's own population 8-d.
+// p is the node's physical index, P the row pitch and Sd the direction stride.
 template <int L, int S>
-__device__ __forceinline__ void pull(const typename Storage<S>::T *src, uint32_t n, uint32_t code,
-                                     uint32_t Hx, uint32_t N, typename Storage<S>::C f[9])
+__device__ __forceinline__ void pull(const typename Storage<S>::T *src, uint32_t p, uint32_t code,
+                                     uint32_t P, uint32_t Sd, typename Storage<S>::C f[9])
 {
 #pragma unroll
     for (int d = 0; d < 9; ++d)
     {
         const bool bb = (code >> d) & 1u;
-        const uint32_t m = bb ? n : uint32_t(int32_t(n) - cy_of(d) * int32_t(Hx) - cx_of(d));
+        const uint32_t m = bb ? p : uint32_t(int32_t(p) - cy_of(d) * int32_t(P) - cx_of(d));
         const int dd = bb ? 8 - d : d;
-        f[d] = Storage<S>::load(src, at<L>(m, uint32_t(dd), N), dd);
+        f[d] = Storage<S>::load(src, at<L>(m, uint32_t(dd), Sd), dd);
     }
 }
 
@@ -204,21 +235,49 @@ __device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C
     }
 }
 
+// In the stream layout every store instruction of a wave writes whole
+// contiguous segments, and the populations are written with non-temporal
+// stores: nothing reads them again within the step, so they should not
+// displace the pull's halo lines in L2.  In the collision and bundle layouts
+// one instruction writes every 9th (3rd) element; those stores go through L2,
+// where the nine (three) instructions of a node row combine into full lines.
 template <int L, int S>
-__device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t n, uint32_t N,
+__device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t p, uint32_t Sd,
                                            const typename Storage<S>::C f[9])
 {
 #pragma unroll
     for (int d = 0; d < 9; ++d)
-        Storage<S>::store(dst, at<L>(n, uint32_t(d), N), f[d], d);
+    {
+        if constexpr (L == LBM_LAYOUT_STREAM)
+            Storage<S>::store_nt(dst, at<L>(p, uint32_t(d), Sd), f[d], d);
+        else
+            Storage<S>::store(dst, at<L>(p, uint32_t(d), Sd), f[d], d);
+    }
+}
+
+// Full-segment stores.  HBM is written in 64-byte segments; a segment that a
+// wave writes only in part (a fluid/solid boundary inside it) costs a read-
+// modify-write.  Solid nodes' populations are never read (half-way bounce-back
+// takes the fluid node's own opposite population instead), so a solid lane
+// writes zeros whenever another lane of its 64-byte segment writes, in the
+// same store instructions as the fluid lanes (one store per direction after
+// the branches reconverge, never one per branch).
+// Ghost, buffer and padding nodes never write: they hold boundary data.
+template <class T>
+__device__ __forceinline__ bool segment_has(bool pred)
+{
+    constexpr int G = 64 / int(sizeof(T));  // lanes per 64-byte segment of one direction array
+    const unsigned long long m = __ballot(pred);
+    const int lane = __lane_id();
+    const unsigned long long group = (G >= 64) ? ~0ull : ((1ull << G) - 1ull);
+    return (m >> (lane & ~(G - 1))) & group;
 }
 
 // Finish one fluid node whose pulled populations are in f: outlet, collision,
-// store, outlet-uy history and (on frame steps) macroscopic outputs at the
-// inner index of unexpanded (x, y).
+// outlet-uy history and (on frame steps) macroscopic outputs at the inner
+// index of unexpanded (x, y).  The caller stores f.
 template <int L, int S>
-__device__ __forceinline__ void finish_node(const StepParams &p, typename Storage<S>::T *dst, uint32_t n,
-                                            uint32_t code, uint32_t x, uint32_t y,
+__device__ __forceinline__ void finish_node(const StepParams &p, uint32_t code, uint32_t x, uint32_t y,
                                             typename Storage<S>::C f[9])
 {
     using C = typename Storage<S>::C;
@@ -226,7 +285,6 @@ __device__ __forceinline__ void finish_node(const StepParams &p, typename Storag
         zou_he_outlet<C>(f, C(p.rho_out), C(p.uy_in[y]));
     C rho, jx, jy, usq;
     collide<C>(f, C(p.omega), rho, jx, jy, usq);
-    store_node<L, S>(dst, n, p.N, f);
     if (code & LBM_CODE_UY_SOURCE)
         p.uy_out[y] = double(clamp_output(jy, p.nan_protection));
     if (p.write_macros)

@@ -48,6 +48,8 @@ struct lbm_ctx
 {
     lbm_desc desc;
     uint32_t H, V, Hx, Vx, N;
+    uint32_t P, X0, Sd;       // physical placement: pitch, row offset, direction stride
+    size_t lattice_elems;     // stored populations per lattice
     uint32_t sub_h, sub_v, cnt_h, cnt_v, wg;
     bool buffered;
     bool two_lattice;
@@ -103,13 +105,19 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
     const uint32_t y = 1u + blockIdx.y;
     if (x > p.Hx - 2u)
         return;
-    const uint32_t n = x + y * p.Hx;
-    const uint32_t code = p.code[n];
-    if (!(code & LBM_CODE_FLUID))
-        return;
-    C f[9];
-    pull<L, S>(static_cast<const T *>(p.src), n, code, p.Hx, p.N, f);
-    finish_node<L, S>(p, static_cast<T *>(p.dst), n, code, x, y, f);
+    const uint32_t pn = place(x, y, p.P, p.X0);
+    const uint32_t code = p.code[pn];
+    const bool fluid = code & LBM_CODE_FLUID;
+    const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
+    const bool fill = (code & LBM_CODE_SOLID) && segment;
+    C f[9] = {};
+    if (fluid)
+    {
+        pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f);
+        finish_node<L, S>(p, code, x, y, f);
+    }
+    if (fluid || fill)
+        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f);
 }
 
 // lptl: two lattices, 1-D range over every node of the unexpanded lattice
@@ -122,12 +130,20 @@ __global__ __launch_bounds__(1024) void k_step_lptl(StepParams p)
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= p.N)
         return;
-    const uint32_t code = p.code[n];
-    if (!(code & LBM_CODE_FLUID))
-        return;
-    C f[9];
-    pull<L, S>(static_cast<const T *>(p.src), n, code, p.Hx, p.N, f);
-    finish_node<L, S>(p, static_cast<T *>(p.dst), n, code, n % p.H, n / p.H, f);
+    const uint32_t x = n % p.H, y = n / p.H;
+    const uint32_t pn = place(x, y, p.P, p.X0);
+    const uint32_t code = p.code[pn];
+    const bool fluid = code & LBM_CODE_FLUID;
+    const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
+    const bool fill = (code & LBM_CODE_SOLID) && segment;
+    C f[9] = {};
+    if (fluid)
+    {
+        pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f);
+        finish_node<L, S>(p, code, x, y, f);
+    }
+    if (fluid || fill)
+        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f);
 }
 
 // npol: one lattice, one work-group per subdomain tile; every pull of the
@@ -142,15 +158,19 @@ __global__ __launch_bounds__(1024) void k_step_npol(StepParams p)
     T *lat = static_cast<T *>(p.dst);
     const uint32_t x = blockIdx.x * (p.sub_h + 1u) + threadIdx.x + 1u;
     const uint32_t y = blockIdx.y * (p.sub_v + 1u) + threadIdx.y + 1u;
-    const uint32_t n = x + y * p.Hx;
-    const uint32_t code = p.code[n];
+    const uint32_t pn = place(x, y, p.P, p.X0);
+    const uint32_t code = p.code[pn];
     const bool fluid = code & LBM_CODE_FLUID;
-    C f[9];
+    const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
+    const bool fill = (code & LBM_CODE_SOLID) && segment;
+    C f[9] = {};
     if (fluid)
-        pull<L, S>(lat, n, code, p.Hx, p.N, f);
+        pull<L, S>(lat, pn, code, p.P, p.S, f);
     __syncthreads();
     if (fluid)
-        finish_node<L, S>(p, lat, n, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
+        finish_node<L, S>(p, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
+    if (fluid || fill)
+        store_node<L, S>(lat, pn, p.S, f);
 }
 
 // nsol: one lattice, swap scheme (nsol_kernels.hpp:171-334).  Work-groups of
@@ -168,50 +188,52 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
     C *lds = reinterpret_cast<C *>(lds_raw);
     T *lat = static_cast<T *>(p.dst);
 
-    const uint32_t P = p.sub_h + 4u;
-    const uint32_t plane = (p.sub_v + 2u) * P;
+    const uint32_t LP = p.sub_h + 4u;
+    const uint32_t plane = (p.sub_v + 2u) * LP;
     C *s5 = lds, *s6 = lds + plane, *s7 = lds + 2u * plane, *s8 = lds + 3u * plane;
 
     const uint32_t lx = threadIdx.x + 1u, ly = threadIdx.y;
     const uint32_t x = blockIdx.x * (p.sub_h + 1u) + threadIdx.x;
     const uint32_t y = blockIdx.y * (p.sub_v + 1u) + threadIdx.y;
-    const uint32_t n = x + y * p.Hx;
-    const uint32_t N = p.N, Hx = p.Hx;
-    const uint32_t code = p.code[n];
+    const uint32_t pn = place(x, y, p.P, p.X0);
+    const uint32_t Sd = p.S, P = p.P;
+    const uint32_t code = p.code[pn];
     const bool fluid = code & LBM_CODE_FLUID;
+    const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
+    const bool fill = (code & LBM_CODE_SOLID) && segment;
 
-    C f[9];
+    C f[9] = {};
     if (!(code & LBM_CODE_SOLID))
     {
-        const C o5 = Storage<S>::load(lat, at<L>(n, 5, N), 5);
-        const C o6 = Storage<S>::load(lat, at<L>(n, 6, N), 6);
-        const C o7 = Storage<S>::load(lat, at<L>(n, 7, N), 7);
-        const C o8 = Storage<S>::load(lat, at<L>(n, 8, N), 8);
-        s5[ly * P + lx + 1u] = o5;
-        s6[(ly + 1u) * P + lx - 1u] = o6;
-        s7[(ly + 1u) * P + lx] = o7;
-        s8[(ly + 1u) * P + lx + 1u] = o8;
+        const C o5 = Storage<S>::load(lat, at<L>(pn, 5, Sd), 5);
+        const C o6 = Storage<S>::load(lat, at<L>(pn, 6, Sd), 6);
+        const C o7 = Storage<S>::load(lat, at<L>(pn, 7, Sd), 7);
+        const C o8 = Storage<S>::load(lat, at<L>(pn, 8, Sd), 8);
+        s5[ly * LP + lx + 1u] = o5;
+        s6[(ly + 1u) * LP + lx - 1u] = o6;
+        s7[(ly + 1u) * LP + lx] = o7;
+        s8[(ly + 1u) * LP + lx + 1u] = o8;
         if (fluid)
         {
-            f[4] = Storage<S>::load(lat, at<L>(n, 4, N), 4);
-            f[3] = (code & (1u << 3)) ? o5 : Storage<S>::load(lat, at<L>(n + 1u, 3, N), 3);
-            f[2] = (code & (1u << 2)) ? o6 : Storage<S>::load(lat, at<L>(n - 1u + Hx, 2, N), 2);
-            f[1] = (code & (1u << 1)) ? o7 : Storage<S>::load(lat, at<L>(n + Hx, 1, N), 1);
-            f[0] = (code & (1u << 0)) ? o8 : Storage<S>::load(lat, at<L>(n + 1u + Hx, 0, N), 0);
+            f[4] = Storage<S>::load(lat, at<L>(pn, 4, Sd), 4);
+            f[3] = (code & (1u << 3)) ? o5 : Storage<S>::load(lat, at<L>(pn + 1u, 3, Sd), 3);
+            f[2] = (code & (1u << 2)) ? o6 : Storage<S>::load(lat, at<L>(pn - 1u + P, 2, Sd), 2);
+            f[1] = (code & (1u << 1)) ? o7 : Storage<S>::load(lat, at<L>(pn + P, 1, Sd), 1);
+            f[0] = (code & (1u << 0)) ? o8 : Storage<S>::load(lat, at<L>(pn + 1u + P, 0, Sd), 0);
             if (code & (1u << 5))
-                f[5] = Storage<S>::load(lat, at<L>(n, 3, N), 3);
+                f[5] = Storage<S>::load(lat, at<L>(pn, 3, Sd), 3);
             if (code & (1u << 6))
-                f[6] = Storage<S>::load(lat, at<L>(n, 2, N), 2);
+                f[6] = Storage<S>::load(lat, at<L>(pn, 2, Sd), 2);
             if (code & (1u << 7))
-                f[7] = Storage<S>::load(lat, at<L>(n, 1, N), 1);
+                f[7] = Storage<S>::load(lat, at<L>(pn, 1, Sd), 1);
             if (code & (1u << 8))
-                f[8] = Storage<S>::load(lat, at<L>(n, 0, N), 0);
+                f[8] = Storage<S>::load(lat, at<L>(pn, 0, Sd), 0);
         }
     }
     __syncthreads();
     if (fluid)
     {
-        const uint32_t own = ly * P + lx;
+        const uint32_t own = ly * LP + lx;
         if (!(code & (1u << 5)))
             f[5] = s5[own];
         if (!(code & (1u << 6)))
@@ -220,60 +242,73 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
             f[7] = s7[own];
         if (!(code & (1u << 8)))
             f[8] = s8[own];
-        finish_node<L, S>(p, lat, n, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
+        finish_node<L, S>(p, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
     }
+    if (fluid || fill)
+        store_node<L, S>(lat, pn, p.S, f);
 }
 
 // Buffer rows: directions 0..2 from the row above, 6..8 from the row below
 // (general/buffered.hpp:74-93).  Covers the full expanded width, ghost
 // columns included.
 template <int L, class T>
-__global__ void k_refresh_rows(T *lat, uint32_t Hx, uint32_t N, uint32_t sub_v)
+__global__ void k_refresh_rows(T *lat, uint32_t Hx, uint32_t P, uint32_t X0, uint32_t Sd, uint32_t sub_v)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= Hx)
         return;
     const uint32_t yb = (blockIdx.y + 1u) * (sub_v + 1u);
-    const uint32_t n = x + yb * Hx;
+    const uint32_t n = place(x, yb, P, X0);
 #pragma unroll
     for (uint32_t d = 0; d < 3; ++d)
-        lat[at<L>(n, d, N)] = lat[at<L>(n + Hx, d, N)];
+        lat[at<L>(n, d, Sd)] = lat[at<L>(n + P, d, Sd)];
 #pragma unroll
     for (uint32_t d = 6; d < 9; ++d)
-        lat[at<L>(n, d, N)] = lat[at<L>(n - Hx, d, N)];
+        lat[at<L>(n, d, Sd)] = lat[at<L>(n - P, d, Sd)];
 }
 
 // Buffer columns: directions 2, 5, 8 from the left, 0, 3, 6 from the right
 // (general/buffered.hpp:133-152).  Runs after the rows so that the corner
 // nodes carry the diagonal neighbours' values.
 template <int L, class T>
-__global__ void k_refresh_cols(T *lat, uint32_t Hx, uint32_t Vx, uint32_t N, uint32_t sub_h)
+__global__ void k_refresh_cols(T *lat, uint32_t Vx, uint32_t P, uint32_t X0, uint32_t Sd, uint32_t sub_h)
 {
     const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= Vx)
         return;
     const uint32_t xb = (blockIdx.y + 1u) * (sub_h + 1u);
-    const uint32_t n = xb + y * Hx;
+    const uint32_t n = place(xb, y, P, X0);
 #pragma unroll
     for (uint32_t d = 2; d < 9; d += 3)
-        lat[at<L>(n, d, N)] = lat[at<L>(n - 1u, d, N)];
+        lat[at<L>(n, d, Sd)] = lat[at<L>(n - 1u, d, Sd)];
 #pragma unroll
     for (uint32_t d = 0; d < 9; d += 3)
-        lat[at<L>(n, d, N)] = lat[at<L>(n + 1u, d, N)];
+        lat[at<L>(n, d, Sd)] = lat[at<L>(n + 1u, d, Sd)];
 }
 
 // ---------------------------------------------------------------------------
 // set-up kernels
 // ---------------------------------------------------------------------------
+struct Geom
+{
+    uint32_t Hx, N, P, X0, Sd;
+};
+
+__device__ __forceinline__ uint32_t place_logical(uint32_t n, const Geom &g)
+{
+    return place(n % g.Hx, n / g.Hx, g.P, g.X0);
+}
+
 template <int L, int S>
-__global__ void k_init_rest(typename Storage<S>::T *lat, uint32_t N)
+__global__ void k_init_rest(typename Storage<S>::T *lat, Geom g)
 {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N)
+    if (n >= g.N)
         return;
+    const uint32_t pn = place_logical(n, g);
 #pragma unroll
     for (int d = 0; d < 9; ++d)
-        lat[at<L>(n, uint32_t(d), N)] = Storage<S>::encode(weight<double>(d), d);
+        lat[at<L>(pn, uint32_t(d), g.Sd)] = Storage<S>::encode(weight<double>(d), d);
 }
 
 struct InoutValues
@@ -284,8 +319,8 @@ struct InoutValues
 // Inlet ghost column x = 0 and outlet ghost column x = H-1, rows 1..V-2, at
 // their equilibria (domain_initialization.hpp:382-398).
 template <int L, int S>
-__global__ void k_init_inout(typename Storage<S>::T *lat, uint32_t H, uint32_t V, uint32_t Hx, uint32_t N,
-                             uint32_t sub_h, uint32_t sub_v, int buffered, InoutValues vals)
+__global__ void k_init_inout(typename Storage<S>::T *lat, uint32_t H, uint32_t V, Geom g, uint32_t sub_h,
+                             uint32_t sub_v, int buffered, InoutValues vals)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t side = blockIdx.y;
@@ -293,16 +328,17 @@ __global__ void k_init_inout(typename Storage<S>::T *lat, uint32_t H, uint32_t V
         return;
     const uint32_t x = 1u + side * (H - 3u);
     const uint32_t y = 1u + j;
-    uint32_t node = buffered ? (x + (x - 1u) / sub_h) + (y + (y - 1u) / sub_v) * Hx : x + y * Hx;
-    node = side ? node + 1u : node - 1u;
+    const uint32_t xe = buffered ? x + (x - 1u) / sub_h : x;
+    const uint32_t ye = buffered ? y + (y - 1u) / sub_v : y;
+    const uint32_t node = place(side ? xe + 1u : xe - 1u, ye, g.P, g.X0);
 #pragma unroll
     for (int d = 0; d < 9; ++d)
-        lat[at<L>(node, uint32_t(d), N)] = Storage<S>::encode(vals.v[side * 9 + d], d);
+        lat[at<L>(node, uint32_t(d), g.Sd)] = Storage<S>::encode(vals.v[side * 9 + d], d);
 }
 
 struct CodeParams
 {
-    uint32_t H, V, Hx, N, sub_h, sub_v;
+    uint32_t H, V, Hx, N, sub_h, sub_v, P, X0;
     int buffered;
     int outlet_active;
 };
@@ -320,6 +356,8 @@ __device__ __forceinline__ uint32_t map_axis(uint32_t u, uint32_t last, uint32_t
     return u + (u - 1u) / sub;
 }
 
+// Codes of every logical node n, stored at its physical place.  Padding
+// places keep 0 (not fluid, not solid).
 __global__ void k_build_codes(const int8_t *phase, uint16_t *code, CodeParams cp)
 {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -327,11 +365,11 @@ __global__ void k_build_codes(const int8_t *phase, uint16_t *code, CodeParams cp
         return;
     const int8_t ph = phase[n];
     uint32_t c = 0;
+    const uint32_t xe = n % cp.Hx, ye = n / cp.Hx;
     if (ph == 1)
         c = LBM_CODE_SOLID;
     else if (ph == 0)
     {
-        const uint32_t xe = n % cp.Hx, ye = n / cp.Hx;
         const uint32_t x = cp.buffered ? xe - xe / (cp.sub_h + 1u) : xe;
         const uint32_t y = cp.buffered ? ye - ye / (cp.sub_v + 1u) : ye;
         c = LBM_CODE_FLUID;
@@ -351,7 +389,14 @@ __global__ void k_build_codes(const int8_t *phase, uint16_t *code, CodeParams cp
         if (x + 3u == cp.H && y >= 2u && y + 3u <= cp.V)
             c |= LBM_CODE_UY_SOURCE;
     }
-    code[n] = uint16_t(c);
+    code[place(xe, ye, cp.P, cp.X0)] = uint16_t(c);
+}
+
+__global__ void k_gather_codes(const uint16_t *code, uint16_t *out, Geom g)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < g.N)
+        out[n] = code[place_logical(n, g)];
 }
 
 __global__ void k_fill(double *p, size_t n, double v)
@@ -361,46 +406,65 @@ __global__ void k_fill(double *p, size_t n, double v)
         p[i] = v;
 }
 
+// logical flat index i of a layout (reference order over N nodes) ->
+// (node, direction)
 template <int L>
-__device__ __forceinline__ int dir_of_flat(uint32_t i, uint32_t N)
+__device__ __forceinline__ void split_flat(uint32_t i, uint32_t N, uint32_t &n, uint32_t &d)
 {
     if constexpr (L == LBM_LAYOUT_COLLISION)
-        return int(i % 9u);
+    {
+        n = i / 9u;
+        d = i % 9u;
+    }
     else if constexpr (L == LBM_LAYOUT_STREAM)
-        return int(i / N);
+    {
+        d = i / N;
+        n = i % N;
+    }
     else
-        return int(3u * (i / (3u * N)) + (i % (3u * N)) % 3u);
+    {
+        const uint32_t g = i / (3u * N), r = i % (3u * N);
+        n = r / 3u;
+        d = 3u * g + r % 3u;
+    }
 }
 
 template <int L, int S>
-__global__ void k_decode(const typename Storage<S>::T *lat, double *out, uint32_t N)
+__global__ void k_decode(const typename Storage<S>::T *lat, double *out, Geom g)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 9u * N)
+    if (i >= 9u * g.N)
         return;
-    out[i] = Storage<S>::decode(lat[i], dir_of_flat<L>(i, N));
+    uint32_t n, d;
+    split_flat<L>(i, g.N, n, d);
+    out[i] = Storage<S>::decode(lat[at<L>(place_logical(n, g), d, g.Sd)], int(d));
 }
 
 template <int L, int S>
-__global__ void k_encode(typename Storage<S>::T *lat, const double *in, uint32_t N)
+__global__ void k_encode(typename Storage<S>::T *lat, const double *in, Geom g)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 9u * N)
+    if (i >= 9u * g.N)
         return;
-    lat[i] = Storage<S>::encode(in[i], dir_of_flat<L>(i, N));
+    uint32_t n, d;
+    split_flat<L>(i, g.N, n, d);
+    lat[at<L>(place_logical(n, g), d, g.Sd)] = Storage<S>::encode(in[i], int(d));
 }
 
 // Equilibrium of a given (rho, j) field on every fluid node (warm start).
 template <int L, int S>
 __global__ void k_set_equilibrium(typename Storage<S>::T *lat, const uint16_t *code, const double *rho,
-                                  const double *jx, const double *jy, uint32_t N, uint32_t Hx, uint32_t H,
-                                  uint32_t sub_h, uint32_t sub_v, int buffered)
+                                  const double *jx, const double *jy, Geom g, uint32_t H, uint32_t sub_h,
+                                  uint32_t sub_v, int buffered)
 {
     using C = typename Storage<S>::C;
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N || !(code[n] & LBM_CODE_FLUID))
+    if (n >= g.N)
         return;
-    const uint32_t xe = n % Hx, ye = n / Hx;
+    const uint32_t pn = place_logical(n, g);
+    if (!(code[pn] & LBM_CODE_FLUID))
+        return;
+    const uint32_t xe = n % g.Hx, ye = n / g.Hx;
     const uint32_t x = buffered ? xe - xe / (sub_h + 1u) : xe;
     const uint32_t y = buffered ? ye - ye / (sub_v + 1u) : ye;
     const uint32_t r = (x - 1u) + (y - 1u) * (H - 2u);
@@ -411,7 +475,7 @@ __global__ void k_set_equilibrium(typename Storage<S>::T *lat, const uint16_t *c
     {
         const C cu = C(cx_of(d)) * ux + C(cy_of(d)) * uy;
         const C feq = weight<C>(d) * (rh + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
-        Storage<S>::store(lat, at<L>(n, uint32_t(d), N), feq, d);
+        Storage<S>::store(lat, at<L>(pn, uint32_t(d), g.Sd), feq, d);
     }
 }
 
@@ -456,18 +520,19 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[K], double *out)
 // mode 1: residual sums |j - j_snap|^2, |j|^2 and snapshot update.
 template <int L, int S>
 __global__ __launch_bounds__(RED_BLOCK) void k_reduce(const typename Storage<S>::T *lat, const uint16_t *code,
-                                                      uint32_t N, double *partials, double *snap, int mode)
+                                                      Geom g, double *partials, double *snap, int mode)
 {
     double acc[RED_K] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t n = blockIdx.x * RED_BLOCK + threadIdx.x; n < N; n += RED_GRID * RED_BLOCK)
+    for (uint32_t n = blockIdx.x * RED_BLOCK + threadIdx.x; n < g.N; n += RED_GRID * RED_BLOCK)
     {
-        const uint32_t c = code[n];
+        const uint32_t pn = place_logical(n, g);
+        const uint32_t c = code[pn];
         if (!(c & LBM_CODE_FLUID))
             continue;
         double f[9];
 #pragma unroll
         for (int d = 0; d < 9; ++d)
-            f[d] = Storage<S>::decode(lat[at<L>(n, uint32_t(d), N)], d);
+            f[d] = Storage<S>::decode(lat[at<L>(pn, uint32_t(d), g.Sd)], d);
         double rho = 0, jx = 0, jy = 0;
 #pragma unroll
         for (int d = 0; d < 9; ++d)
@@ -518,7 +583,8 @@ __global__ void k_reduce_final(const double *partials, int blocks, double *out)
 
 __global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
 {
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
         b[i] = a[i];
 }
 
@@ -551,6 +617,9 @@ struct Impl
         p.Hx = c->Hx;
         p.Vx = c->Vx;
         p.N = c->N;
+        p.P = c->P;
+        p.X0 = c->X0;
+        p.S = c->Sd;
         p.sub_h = c->sub_h;
         p.sub_v = c->sub_v;
         p.write_macros = macros ? 1u : 0u;
@@ -596,17 +665,20 @@ struct Impl
         T *lat = static_cast<T *>(c->lat[c->cur]);
         if (c->cnt_v > 1)
             hipLaunchKernelGGL((k_refresh_rows<L, T>), dim3((c->Hx + 255u) / 256u, c->cnt_v - 1u), dim3(256), 0,
-                               c->stream, lat, c->Hx, c->N, c->sub_v);
+                               c->stream, lat, c->Hx, c->P, c->X0, c->Sd, c->sub_v);
         if (c->cnt_h > 1)
             hipLaunchKernelGGL((k_refresh_cols<L, T>), dim3((c->Vx + 255u) / 256u, c->cnt_h - 1u), dim3(256), 0,
-                               c->stream, lat, c->Hx, c->Vx, c->N, c->sub_h);
+                               c->stream, lat, c->Vx, c->P, c->X0, c->Sd, c->sub_h);
     }
+
+    static Geom geom(const lbm_ctx *c) { return Geom{c->Hx, c->N, c->P, c->X0, c->Sd}; }
 
     static int init(lbm_ctx *c)
     {
         const uint32_t blocks = (c->N + 255u) / 256u;
         T *l0 = static_cast<T *>(c->lat[0]);
-        hipLaunchKernelGGL((k_init_rest<L, S>), dim3(blocks), dim3(256), 0, c->stream, l0, c->N);
+        LBM_HIP_TRY(hipMemsetAsync(l0, 0, c->lattice_elems * sizeof(T), c->stream));
+        hipLaunchKernelGGL((k_init_rest<L, S>), dim3(blocks), dim3(256), 0, c->stream, l0, geom(c));
         InoutValues v;
         for (int d = 0; d < 9; ++d)
         {
@@ -615,10 +687,10 @@ struct Impl
         }
         if (c->V > 2u)
             hipLaunchKernelGGL((k_init_inout<L, S>), dim3((c->V - 2u + 255u) / 256u, 2), dim3(256), 0, c->stream,
-                               l0, c->H, c->V, c->Hx, c->N, c->sub_h, c->sub_v, c->buffered ? 1 : 0, v);
+                               l0, c->H, c->V, geom(c), c->sub_h, c->sub_v, c->buffered ? 1 : 0, v);
         LBM_HIP_TRY(hipGetLastError());
         if (c->two_lattice)
-            LBM_HIP_TRY(hipMemcpyAsync(c->lat[1], c->lat[0], 9ull * c->N * sizeof(T), hipMemcpyDeviceToDevice,
+            LBM_HIP_TRY(hipMemcpyAsync(c->lat[1], c->lat[0], c->lattice_elems * sizeof(T), hipMemcpyDeviceToDevice,
                                        c->stream));
         return LBM_OK;
     }
@@ -627,7 +699,7 @@ struct Impl
     {
         const uint32_t total = 9u * c->N;
         hipLaunchKernelGGL((k_decode<L, S>), dim3((total + 255u) / 256u), dim3(256), 0, c->stream,
-                           static_cast<const T *>(c->lat[c->cur]), dev_out, c->N);
+                           static_cast<const T *>(c->lat[c->cur]), dev_out, geom(c));
         LBM_HIP_TRY(hipGetLastError());
         return LBM_OK;
     }
@@ -636,7 +708,7 @@ struct Impl
     {
         const uint32_t total = 9u * c->N;
         hipLaunchKernelGGL((k_encode<L, S>), dim3((total + 255u) / 256u), dim3(256), 0, c->stream,
-                           static_cast<T *>(c->lat[c->cur]), dev_in, c->N);
+                           static_cast<T *>(c->lat[c->cur]), dev_in, geom(c));
         LBM_HIP_TRY(hipGetLastError());
         return LBM_OK;
     }
@@ -644,7 +716,7 @@ struct Impl
     static int equilibrium(lbm_ctx *c, const double *rho, const double *jx, const double *jy)
     {
         hipLaunchKernelGGL((k_set_equilibrium<L, S>), dim3((c->N + 255u) / 256u), dim3(256), 0, c->stream,
-                           static_cast<T *>(c->lat[c->cur]), c->code, rho, jx, jy, c->N, c->Hx, c->H, c->sub_h,
+                           static_cast<T *>(c->lat[c->cur]), c->code, rho, jx, jy, geom(c), c->H, c->sub_h,
                            c->sub_v, c->buffered ? 1 : 0);
         LBM_HIP_TRY(hipGetLastError());
         return LBM_OK;
@@ -653,7 +725,7 @@ struct Impl
     static int reduce(lbm_ctx *c, int mode, double *dev_out6)
     {
         hipLaunchKernelGGL((k_reduce<L, S>), dim3(RED_GRID), dim3(RED_BLOCK), 0, c->stream,
-                           static_cast<const T *>(c->lat[c->cur]), c->code, c->N, c->partials, c->snap, mode);
+                           static_cast<const T *>(c->lat[c->cur]), c->code, geom(c), c->partials, c->snap, mode);
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, c->partials, RED_GRID, dev_out6);
         LBM_HIP_TRY(hipGetLastError());
         return LBM_OK;
@@ -820,6 +892,21 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     c->outlet_active = !(c->buffered && (c->H - 3u) % c->sub_h == 0u);
     c->elem = storage_bytes(desc->storage);
     c->D = (c->H - 2u) * (c->V - 2u);
+    // physical placement (see Placement in lbm_device.hpp)
+    c->X0 = kRowOffset;
+    c->P = (c->X0 + c->Hx + 63u) / 64u * 64u;
+    {
+        const uint64_t M = uint64_t(c->P) * c->Vx;
+        const uint64_t align_nodes = (M * c->elem >= (32ull << 20) ? (2ull << 20) : 4096ull) / c->elem;
+        const uint64_t Sd = (M + align_nodes - 1) / align_nodes * align_nodes;
+        if (9ull * Sd >= (1ull << 32))
+        {
+            delete c;
+            return fail(LBM_ERR_INVALID_ARGUMENT, "lattice too large for 32-bit population indices");
+        }
+        c->Sd = uint32_t(Sd);
+        c->lattice_elems = size_t(9) * Sd;
+    }
     host_equilibrium(desc->inlet_velocity_x, desc->inlet_velocity_y, desc->inlet_density, c->feq_in);
     host_equilibrium(desc->outlet_velocity_x, desc->outlet_velocity_y, desc->outlet_density, c->feq_out);
 
@@ -839,12 +926,12 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     CREATE_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     CREATE_TRY(hipEventCreate(&c->ev0));
     CREATE_TRY(hipEventCreate(&c->ev1));
-    const size_t lat_bytes = 9ull * c->N * c->elem;
+    const size_t lat_bytes = c->lattice_elems * c->elem;
     CREATE_TRY(hipMalloc(&c->lat[0], lat_bytes));
     if (c->two_lattice)
         CREATE_TRY(hipMalloc(&c->lat[1], lat_bytes));
     CREATE_TRY(hipMalloc(&c->phase, c->N));
-    CREATE_TRY(hipMalloc(&c->code, size_t(c->N) * sizeof(uint16_t)));
+    CREATE_TRY(hipMalloc(&c->code, size_t(c->Sd) * sizeof(uint16_t)));
     CREATE_TRY(hipMalloc(&c->res, 4ull * std::max(1u, c->D) * sizeof(double)));
     CREATE_TRY(hipMalloc(&c->uycol, 2ull * c->V * sizeof(double)));
     CREATE_TRY(hipMalloc(&c->partials, size_t(RED_GRID) * RED_K * sizeof(double)));
@@ -852,7 +939,7 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     // Data fills phase with -1 (simulation.cpp:424); Results fill density -1,
     // momentum 0 (simulation.cpp:407-410).
     CREATE_TRY(hipMemsetAsync(c->phase, 0xff, c->N, c->stream));
-    CREATE_TRY(hipMemsetAsync(c->code, 0, size_t(c->N) * sizeof(uint16_t), c->stream));
+    CREATE_TRY(hipMemsetAsync(c->code, 0, size_t(c->Sd) * sizeof(uint16_t), c->stream));
     CREATE_TRY(hipMemsetAsync(c->res, 0, 4ull * std::max(1u, c->D) * sizeof(double), c->stream));
     CREATE_TRY(hipMemsetAsync(c->uycol, 0, 2ull * c->V * sizeof(double), c->stream));
     if (c->D)
@@ -899,7 +986,9 @@ int lbm_init_populations(lbm_ctx *c)
     c->cur = 0;
     if (int rc = vtable(c).init(c))
         return rc;
-    CodeParams cp{c->H, c->V, c->Hx, c->N, c->sub_h, c->sub_v, c->buffered ? 1 : 0, c->outlet_active ? 1 : 0};
+    CodeParams cp{c->H,  c->V,  c->Hx, c->N, c->sub_h, c->sub_v, c->P, c->X0, c->buffered ? 1 : 0,
+                  c->outlet_active ? 1 : 0};
+    LBM_HIP_TRY(hipMemsetAsync(c->code, 0, size_t(c->Sd) * sizeof(uint16_t), c->stream));
     hipLaunchKernelGGL(k_build_codes, dim3((c->N + 255u) / 256u), dim3(256), 0, c->stream, c->phase, c->code, cp);
     LBM_HIP_TRY(hipGetLastError());
     LBM_HIP_TRY(hipMemsetAsync(c->uycol, 0, 2ull * c->V * sizeof(double), c->stream));
@@ -918,8 +1007,18 @@ int lbm_get_node_codes(lbm_ctx *c, uint16_t *codes, size_t n)
         return fail(LBM_ERR_STATE, "lbm_get_node_codes: call lbm_init_populations first");
     if (n != c->N)
         return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_node_codes: node_count must equal the expanded node count");
-    LBM_HIP_TRY(hipMemcpyAsync(codes, c->code, n * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
-    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    uint16_t *tmp = nullptr;
+    LBM_HIP_TRY(hipMalloc(&tmp, n * sizeof(uint16_t)));
+    const Geom g{c->Hx, c->N, c->P, c->X0, c->Sd};
+    hipLaunchKernelGGL(k_gather_codes, dim3((c->N + 255u) / 256u), dim3(256), 0, c->stream, c->code, tmp, g);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(codes, tmp, n * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return fail(LBM_ERR_DEVICE, std::string("lbm_get_node_codes: ") + hipGetErrorString(e));
     return LBM_OK;
 }
 
@@ -1014,8 +1113,8 @@ int lbm_set_populations(lbm_ctx *c, const double *values, size_t count)
                              : fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
     if (!rc && c->two_lattice)
     {
-        e = hipMemcpyAsync(c->lat[1 - c->cur], c->lat[c->cur], 9ull * c->N * c->elem, hipMemcpyDeviceToDevice,
-                           c->stream);
+        e = hipMemcpyAsync(c->lat[1 - c->cur], c->lat[c->cur], c->lattice_elems * c->elem,
+                           hipMemcpyDeviceToDevice, c->stream);
         if (e != hipSuccess)
             rc = fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
     }
@@ -1126,8 +1225,8 @@ int lbm_measure_copy_bandwidth(size_t bytes, int repeats, double *gbps)
         for (int r = 0; r <= repeats && rc == LBM_OK; ++r)
         {
             check(hipEventRecord(e0, s), "hipEventRecord");
-            hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, s, static_cast<const uint4 *>(a),
-                               static_cast<uint4 *>(b), n);
+            hipLaunchKernelGGL(k_copy16, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s,
+                               static_cast<const uint4 *>(a), static_cast<uint4 *>(b), n);
             check(hipGetLastError(), "k_copy16");
             check(hipEventRecord(e1, s), "hipEventRecord");
             check(hipEventSynchronize(e1), "hipEventSynchronize");
