@@ -177,6 +177,11 @@ int oracle_domain(int alg, unsigned H, unsigned V, unsigned wg, unsigned out[7])
         if (w & 0x1)
             w--;
         power_of_two_handling(w, &sub_v, &sub_h);
+        /* which_power_of_4 reports 1 for every w = 4m with m = 1 (mod 4)
+         * (e.g. 20), giving 2 x 2 tiles; without the halo columns that is a
+         * tile of width 0, on which the reference divides by zero */
+        if (sub_v < 2 || sub_h < 3)
+            return -1;
         sub_v -= 1;
         sub_h -= 2;
         cnt_v = count_of(V - 2, sub_v);

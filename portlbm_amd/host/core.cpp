@@ -285,6 +285,12 @@ void Domain::setup_for_swap(unsigned int H, unsigned int V, std::size_t wg, cons
     if (wg & 1u)
         --wg;
     power_of_two_handling(wg, subdomain_vertical_nodes, subdomain_horizontal_nodes);
+    // which_power_of_4 reports 1 for every wg = 4m with m = 1 (mod 4), e.g.
+    // 20, which gives 2 x 2 tiles; without the halo columns such a tile has
+    // width 0 (the reference divides by zero here)
+    if (subdomain_vertical_nodes < 2 || subdomain_horizontal_nodes < 3)
+        throw exceptions::json::PropertyArgumentException(
+            "work group size " + std::to_string(wg) + " gives a swap tile of width 0; choose another size");
     subdomain_vertical_nodes -= 1;    // room for the halo row of the work-group
     subdomain_horizontal_nodes -= 2;  // room for the two halo columns
     subdomain_count_vertical = tiles_needed(V - 2, subdomain_vertical_nodes);
