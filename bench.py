@@ -125,6 +125,54 @@ def sweep(out_path: str, steps: int, warmup: int) -> None:
                                           "rows": rows}, indent=1))
 
 
+def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8)) -> None:
+    """Reduced-precision storage against FP64 on the 8192 x 2048 cylinder
+    channel (disc r = 64 at x = 1024, centred), inlet 1.02 / outlet 1.0.
+    Errors are over fluid nodes, relative to the FP64 run of the same tau:
+    L2 = ||a - a64||_2 / ||a64||_2 and Linf = max|a - a64| / max|a64| for the
+    density and the momentum vector j; mass drift = sum rho(T) / sum rho(0) - 1
+    (FP64 sums on the GPU); drag = momentum-exchange force on the disc."""
+    rows = []
+    for tau in taus:
+        ref = None
+        for storage in ("f64", "f32", "f16-shifted", "bf16-shifted"):
+            doc = pa.settings("nptl", "stream", inner_h=INNER_H, inner_v=INNER_V, scenario="cylinder",
+                              work_group_size=256, time_steps=steps, tau=tau, rho_in=1.02, rho_out=1.0,
+                              storage=storage, cylinderX=1024, cylinderRadius=64.0)
+            lat = pa.Lattice.from_settings(doc)
+            try:
+                fluid_mask = None
+                m0 = lat.reduce()["mass"]
+                ms = lat.run(steps, 0)
+                red = lat.reduce()
+                rho, jx, jy, _ = lat.macros()
+            finally:
+                lat.close()
+            fluid = rho > -1
+            row = {"tau": tau, "storage": storage, "steps": steps, "fluid_nodes": int(fluid.sum()),
+                   "mflups": round(int(fluid.sum()) * steps / (ms * 1e-3) / 1e6, 1),
+                   "mass_drift": red["mass"] / m0 - 1.0, "drag_x": red["fx"], "drag_y": red["fy"],
+                   "finite": bool(np.isfinite(rho[fluid]).all() and np.isfinite(jx[fluid]).all())}
+            if storage == "f64":
+                ref = (rho, jx, jy, red)
+            else:
+                r64, jx64, jy64, red64 = ref
+                f = fluid & (r64 > -1)
+                dj = np.hypot(jx[f] - jx64[f], jy[f] - jy64[f])
+                j64 = np.hypot(jx64[f], jy64[f])
+                row.update({
+                    "rho_l2": float(np.linalg.norm(rho[f] - r64[f]) / np.linalg.norm(r64[f])),
+                    "rho_linf": float(np.abs(rho[f] - r64[f]).max() / np.abs(r64[f]).max()),
+                    "u_l2": float(np.linalg.norm(dj) / np.linalg.norm(j64)),
+                    "u_linf": float(dj.max() / j64.max()),
+                    "drag_x_rel": float(abs(red["fx"] - red64["fx"]) / abs(red64["fx"])),
+                })
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+    Path(out_path).write_text(json.dumps({"workload": f"cylinder r=64 at x=1024 in {INNER_H}x{INNER_V}, "
+                                          "rho_in 1.02 / rho_out 1.0", "rows": rows}, indent=1))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +185,8 @@ def main() -> int:
                     help="porous-circles (the benchmark workload) or any other scenario, for comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep", metavar="OUT.json")
+    ap.add_argument("--precision-study", metavar="OUT.json")
+    ap.add_argument("--study-steps", type=int, default=20000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -146,6 +196,8 @@ def main() -> int:
 
     if args.sweep:
         sweep(args.sweep, min(args.steps, 500), min(args.warmup, 20))
+    if args.precision_study:
+        precision_study(args.precision_study, args.study_steps)
 
     doc = workload(args.algorithm, args.layout, args.storage, args.steps, args.scenario)
     r = time_gpu(doc, args.steps, args.warmup)

@@ -120,18 +120,18 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
         store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f);
 }
 
-// lptl: two lattices, 1-D range over every node of the unexpanded lattice
-// (lptl.hpp:63-65), work-group size from the settings.
+// lptl: two lattices, one linear range over every node of the lattice
+// (lptl.hpp:63-65), work-group size from the settings.  The range runs over
+// the padded placement, so the thread index is the node's physical index and
+// waves stay aligned with rows; padding places carry code 0 and are skipped.
 template <int L, int S>
 __global__ __launch_bounds__(1024) void k_step_lptl(StepParams p)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= p.N)
+    const uint32_t pn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pn >= p.P * p.Vx)
         return;
-    const uint32_t x = n % p.H, y = n / p.H;
-    const uint32_t pn = place(x, y, p.P, p.X0);
     const uint32_t code = p.code[pn];
     const bool fluid = code & LBM_CODE_FLUID;
     const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
@@ -139,6 +139,7 @@ __global__ __launch_bounds__(1024) void k_step_lptl(StepParams p)
     C f[9] = {};
     if (fluid)
     {
+        const uint32_t y = pn / p.P, x = pn - y * p.P - p.X0;
         pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f);
         finish_node<L, S>(p, code, x, y, f);
     }
@@ -636,7 +637,8 @@ struct Impl
         case LBM_ALG_LPTL:
         {
             const uint32_t b = c->wg;
-            hipLaunchKernelGGL((k_step_lptl<L, S>), dim3((c->N + b - 1u) / b), dim3(b), 0, c->stream, p);
+            const uint32_t range = c->P * c->Vx;
+            hipLaunchKernelGGL((k_step_lptl<L, S>), dim3((range + b - 1u) / b), dim3(b), 0, c->stream, p);
             break;
         }
         case LBM_ALG_NPOL:
