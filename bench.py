@@ -173,6 +173,37 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8)) -> None:
                                           "rho_in 1.02 / rho_out 1.0", "rows": rows}, indent=1))
 
 
+def batched(out_path: str, steps: int, count: int = 512) -> dict:
+    """512 uneven porous channels in FP32 (portlbm_amd.workloads.batch_channels):
+    per-domain launches vs one persistent launch per step vs the whole run in
+    one hipGraph with autotuned tiles."""
+    from portlbm_amd import workloads
+
+    docs = workloads.batch_channels(count, steps=steps)
+    rows = {}
+    with pa.Batch(docs, storage="f32") as b:
+        fluid = b.fluid_nodes()
+        nodes = sum((d["domain"]["horizontalNodes"]) * (d["domain"]["verticalNodes"]) for d in docs)
+        for mode in ("per-domain", "persistent"):
+            b.init()
+            ms = b.run(steps, mode)
+            rows[mode] = {"ms": round(ms, 2), "mflups": round(fluid * steps / (ms * 1e-3) / 1e6, 1)}
+            print(json.dumps({mode: rows[mode]}), file=sys.stderr, flush=True)
+        tune_ms = b.autotune(steps_per_trial=20)
+        b.run(steps, "graph")  # capture + first replay
+        b.init()
+        ms = b.run(steps, "graph")
+        rows["graph+autotune"] = {"ms": round(ms, 2), "mflups": round(fluid * steps / (ms * 1e-3) / 1e6, 1),
+                                  "tile_widths_by_class": b.tiles(), "autotune_best_ms_per_step": round(tune_ms, 4)}
+        print(json.dumps({"graph+autotune": rows["graph+autotune"]}), file=sys.stderr, flush=True)
+    result = {"workload": f"{count} porous channels nx U[64,1024] ny U[32,256], FP32, {steps} steps",
+              "domains": count, "inner_nodes": nodes, "fluid_nodes": fluid, "modes": rows,
+              "speedup_graph_vs_per_domain": round(rows["per-domain"]["ms"] / rows["graph+autotune"]["ms"], 2),
+              "gbps_graph": round(rows["graph+autotune"]["mflups"] * 1e6 * 72 / 1e9, 1)}
+    Path(out_path).write_text(json.dumps(result, indent=1))
+    return result
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +218,8 @@ def main() -> int:
     ap.add_argument("--sweep", metavar="OUT.json")
     ap.add_argument("--precision-study", metavar="OUT.json")
     ap.add_argument("--study-steps", type=int, default=20000)
+    ap.add_argument("--batched", metavar="OUT.json")
+    ap.add_argument("--batch-steps", type=int, default=2000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -198,6 +231,8 @@ def main() -> int:
         sweep(args.sweep, min(args.steps, 500), min(args.warmup, 20))
     if args.precision_study:
         precision_study(args.precision_study, args.study_steps)
+    if args.batched:
+        batched(args.batched, args.batch_steps)
 
     doc = workload(args.algorithm, args.layout, args.storage, args.steps, args.scenario)
     r = time_gpu(doc, args.steps, args.warmup)

@@ -54,8 +54,8 @@ struct Disc
 // drawn from splitmix64(seed) as 53-bit uniforms in that order.  A candidate
 // is rejected when it leaves [x_lo, x_hi] x [3, V-4] or comes within one node
 // of an accepted disc (centre distance^2 < (r1 + r2 + 1)^2).  Placement stops
-// when the solid node count reaches (1 - porosity) (H-2)(V-4) or after 2^24
-// candidates.  Node (x, y) is solid iff (x-cx)^2 + (y-cy)^2 <= r^2.
+// when the solid node count reaches (1 - porosity) (H-2)(V-4) or after
+// max_attempts candidates.  Node (x, y) is solid iff (x-cx)^2 + (y-cy)^2 <= r^2.
 std::vector<Disc> porous_discs(unsigned int H, unsigned int V, const Extensions &ext);
 
 // Phase table over the expanded lattice of `domain`, for any scenario of

@@ -30,6 +30,7 @@ struct Extensions
     double radius_min = 4.0;
     double radius_max = 16.0;
     unsigned int obstacle_margin = 32;
+    std::uint64_t max_attempts = 1ull << 24;  // disc candidates tried at most
     // "cylinder": disc centred at (cylinder_x, (V-1)/2)
     unsigned int cylinder_x = 1024;
     double cylinder_radius = 64.0;

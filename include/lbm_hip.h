@@ -192,6 +192,54 @@ int lbm_sync(lbm_ctx *ctx);
  * 8 TB/s HBM figure.  No reference counterpart. */
 int lbm_measure_copy_bandwidth(size_t bytes, int repeats, double *gbps);
 
+/* ---- batched domains (no reference counterpart) ------------------------
+ * Many independent channels of different sizes and physics in one arena,
+ * advanced together.  Every domain is an unexpanded lattice (the lptl
+ * geometry: H x V nodes incl. ghosts) in the stream layout with two lattices;
+ * its update is the same pull / bounce-back / outlet / BGK step as above.
+ * Tiles of 256 work-items cover tile_width x (256 / tile_width) nodes; the
+ * tile width is chosen per class of inner width (<= 128, <= 256, <= 512,
+ * > 512) from {64, 128, 256}. */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_batch_domain {
+    uint32_t horizontal_nodes; /* H incl. ghosts */
+    uint32_t vertical_nodes;   /* V incl. ghosts */
+    double relaxation_time;
+    double inlet_density;
+    double inlet_velocity_x;
+    double inlet_velocity_y;
+    double outlet_density;
+    double outlet_velocity_x;
+    double outlet_velocity_y;
+} lbm_batch_domain;
+
+enum lbm_batch_mode {
+    LBM_BATCH_PER_DOMAIN = 0, /* one launch per domain and step */
+    LBM_BATCH_PERSISTENT = 1, /* one launch per step: a fixed grid strides over the tiles of all domains */
+    LBM_BATCH_GRAPH = 2       /* the persistent launches of the whole run captured in one hipGraph */
+};
+
+int lbm_batch_create(const lbm_batch_domain *domains, uint32_t count, int32_t storage, int32_t nan_protection,
+                     lbm_batch **out);
+void lbm_batch_destroy(lbm_batch *b);
+/* phase table of one domain, H * V values, row-major (same values as lbm_set_phase) */
+int lbm_batch_set_phase(lbm_batch *b, uint32_t domain, const int8_t *phase, size_t node_count);
+/* rest state, ghost equilibria, node codes, zero outlet history and step counter */
+int lbm_batch_init(lbm_batch *b);
+/* advance every domain by `steps`; macroscopic outputs are written on the last step */
+int lbm_batch_run(lbm_batch *b, uint32_t steps, int32_t mode);
+int lbm_batch_last_run_ms(lbm_batch *b, double *ms);
+int lbm_batch_get_macros(lbm_batch *b, uint32_t domain, double *density, double *x_velocity, double *y_velocity,
+                         double *absolute_velocity, size_t count);
+int lbm_batch_fluid_nodes(lbm_batch *b, uint64_t *count);
+int lbm_batch_set_tiles(lbm_batch *b, const uint32_t tile_width[4]);
+int lbm_batch_get_tiles(lbm_batch *b, uint32_t tile_width[4]);
+/* Times the persistent step for every candidate tile width of every width
+ * class (coordinate descent, `steps_per_trial` steps each), keeps the fastest
+ * and re-initialises the domains (trials advance the state). */
+int lbm_batch_autotune(lbm_batch *b, uint32_t steps_per_trial, double *best_ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

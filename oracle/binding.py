@@ -22,7 +22,7 @@ _lib: ctypes.CDLL | None = None
 class ScenarioParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("porosity", ctypes.c_double), ("radius_min", ctypes.c_double),
                 ("radius_max", ctypes.c_double), ("margin", ctypes.c_uint), ("cylinder_x", ctypes.c_uint),
-                ("cylinder_radius", ctypes.c_double)]
+                ("cylinder_radius", ctypes.c_double), ("max_attempts", ctypes.c_uint64)]
 
 
 def library() -> ctypes.CDLL:
@@ -113,7 +113,7 @@ class Simulation:
         params = ScenarioParams(int(ext.get("seed", 20261202)), float(ext.get("porosity", 0.7)),
                                 float(ext.get("radiusMin", 4.0)), float(ext.get("radiusMax", 16.0)),
                                 int(ext.get("obstacleMargin", 32)), int(ext.get("cylinderX", 1024)),
-                                float(ext.get("cylinderRadius", 64.0)))
+                                float(ext.get("cylinderRadius", 64.0)), int(ext.get("maxAttempts", 1 << 24)))
         if lib.oracle_build_phase(self._s, SCENARIOS[dom["scenario"]], ctypes.byref(params)) != 0:
             raise ValueError("oracle_build_phase failed")
         lib.oracle_init(self._s)

@@ -429,7 +429,7 @@ typedef struct
  * r ~ U[rmin, rmax), centre x ~ U{x_lo..x_hi}, y ~ U{2..V-3}; rejected when
  * the disc leaves [x_lo, x_hi] x [3, V-4] or comes closer than one node to
  * an accepted disc ((dx^2 + dy^2) < (r1 + r2 + 1)^2).  Accepted until the
- * solid node count reaches (1 - porosity) * (H-2) * (V-4) or 2^24 candidates
+ * solid node count reaches (1 - porosity) * (H-2) * (V-4) or max_attempts candidates
  * were tried.  A node (x, y) is solid iff (x-cx)^2 + (y-cy)^2 <= r^2.
  * This is a plain O(n^2) restatement; the product uses a cell grid. */
 static size_t porous_circles(unsigned H, unsigned V, const oracle_scenario_params *p, circle_t **out_c)
@@ -440,7 +440,7 @@ static size_t porous_circles(unsigned H, unsigned V, const oracle_scenario_param
     size_t cap = 1024, n = 0;
     circle_t *c = (circle_t *)malloc(cap * sizeof(circle_t));
     double solid = 0.0;
-    for (uint64_t attempt = 0; attempt < (1ull << 24) && solid < target && x_hi >= x_lo; ++attempt)
+    for (uint64_t attempt = 0; attempt < p->max_attempts && solid < target && x_hi >= x_lo; ++attempt)
     {
         const double r = p->radius_min + (p->radius_max - p->radius_min) * sm_u01(&st);
         const double cx = x_lo + floor(sm_u01(&st) * (x_hi - x_lo + 1.0));

@@ -52,6 +52,7 @@ typedef struct oracle_scenario_params {
     unsigned margin;
     unsigned cylinder_x;
     double cylinder_radius;
+    uint64_t max_attempts;
 } oracle_scenario_params;
 
 typedef struct oracle_sim oracle_sim;

@@ -493,3 +493,130 @@ class Handler:
         out = np.empty(9 * self.extents["N"])
         _check_host(self.lib.portlbm_get_populations(self._h, _ptr(out), out.size), "get_populations")
         return out
+
+
+# ---------------------------------------------------------------------------
+# batched domains
+# ---------------------------------------------------------------------------
+class BatchDomain(ctypes.Structure):
+    _fields_ = [
+        ("horizontal_nodes", ctypes.c_uint32),
+        ("vertical_nodes", ctypes.c_uint32),
+        ("relaxation_time", ctypes.c_double),
+        ("inlet_density", ctypes.c_double),
+        ("inlet_velocity_x", ctypes.c_double),
+        ("inlet_velocity_y", ctypes.c_double),
+        ("outlet_density", ctypes.c_double),
+        ("outlet_velocity_x", ctypes.c_double),
+        ("outlet_velocity_y", ctypes.c_double),
+    ]
+
+
+BATCH_MODES = {"per-domain": 0, "persistent": 1, "graph": 2}
+_batch_sig_done = False
+
+
+def _batch_sigs() -> ctypes.CDLL:
+    global _batch_sig_done
+    lib = hip_library()
+    if not _batch_sig_done:
+        sig = {
+            "lbm_batch_create": ([ctypes.POINTER(BatchDomain), _U32, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.POINTER(_P)], ctypes.c_int),
+            "lbm_batch_destroy": ([_P], None),
+            "lbm_batch_set_phase": ([_P, _U32, _P, _SZ], ctypes.c_int),
+            "lbm_batch_init": ([_P], ctypes.c_int),
+            "lbm_batch_run": ([_P, _U32, ctypes.c_int32], ctypes.c_int),
+            "lbm_batch_last_run_ms": ([_P, _DP], ctypes.c_int),
+            "lbm_batch_get_macros": ([_P, _U32, _P, _P, _P, _P, _SZ], ctypes.c_int),
+            "lbm_batch_fluid_nodes": ([_P, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+            "lbm_batch_set_tiles": ([_P, _P], ctypes.c_int),
+            "lbm_batch_get_tiles": ([_P, _P], ctypes.c_int),
+            "lbm_batch_autotune": ([_P, _U32, _DP], ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _batch_sig_done = True
+    return lib
+
+
+class Batch:
+    """Many independent channels advanced together (include/lbm_hip.h,
+    lbm_batch_*).  `docs` are settings documents; each domain uses the
+    unexpanded lattice of its document."""
+
+    def __init__(self, docs: list[dict], storage: str = "f32", nan_protection: bool = True):
+        self.lib = _batch_sigs()
+        n = len(docs)
+        arr = (BatchDomain * n)()
+        self.shapes = []
+        for i, doc in enumerate(docs):
+            dom, phy = doc["domain"], doc["physical"]
+            H, V = dom["horizontalNodes"] + 2, dom["verticalNodes"] + 2
+            arr[i] = BatchDomain(H, V, phy["relaxationTime"], phy["inletDensity"], phy["inletVelocity"]["x"],
+                                 phy["inletVelocity"]["y"], phy["outletDensity"], phy["outletVelocity"]["x"],
+                                 phy["outletVelocity"]["y"])
+            self.shapes.append((H, V))
+        self._b = ctypes.c_void_p()
+        _check_hip(self.lib.lbm_batch_create(arr, n, STORAGES[storage], 1 if nan_protection else 0,
+                                             ctypes.byref(self._b)), "lbm_batch_create")
+        for i, doc in enumerate(docs):
+            lin = json.loads(json.dumps(doc))
+            lin["algorithmic"]["algorithm"] = "lptl"  # the unexpanded lattice
+            phase = build_phase(lin)
+            _check_hip(self.lib.lbm_batch_set_phase(self._b, i, _ptr(phase), phase.size), "lbm_batch_set_phase")
+        _check_hip(self.lib.lbm_batch_init(self._b), "lbm_batch_init")
+
+    def close(self) -> None:
+        if self._b:
+            self.lib.lbm_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self) -> None:
+        _check_hip(self.lib.lbm_batch_init(self._b), "lbm_batch_init")
+
+    def run(self, steps: int, mode: str = "graph") -> float:
+        _check_hip(self.lib.lbm_batch_run(self._b, steps, BATCH_MODES[mode]), "lbm_batch_run")
+        ms = ctypes.c_double(0)
+        _check_hip(self.lib.lbm_batch_last_run_ms(self._b, ctypes.byref(ms)), "lbm_batch_last_run_ms")
+        return ms.value
+
+    def macros(self, domain: int):
+        H, V = self.shapes[domain]
+        D = (H - 2) * (V - 2)
+        out = [np.empty(D) for _ in range(4)]
+        _check_hip(self.lib.lbm_batch_get_macros(self._b, domain, *map(_ptr, out), D), "lbm_batch_get_macros")
+        return tuple(out)
+
+    def fluid_nodes(self) -> int:
+        v = ctypes.c_uint64(0)
+        _check_hip(self.lib.lbm_batch_fluid_nodes(self._b, ctypes.byref(v)), "lbm_batch_fluid_nodes")
+        return v.value
+
+    def tiles(self) -> list[int]:
+        out = (ctypes.c_uint32 * 4)()
+        _check_hip(self.lib.lbm_batch_get_tiles(self._b, out), "lbm_batch_get_tiles")
+        return list(out)
+
+    def set_tiles(self, widths: list[int]) -> None:
+        arr = (ctypes.c_uint32 * 4)(*widths)
+        _check_hip(self.lib.lbm_batch_set_tiles(self._b, arr), "lbm_batch_set_tiles")
+
+    def autotune(self, steps_per_trial: int = 20) -> float:
+        best = ctypes.c_double(0)
+        _check_hip(self.lib.lbm_batch_autotune(self._b, steps_per_trial, ctypes.byref(best)), "lbm_batch_autotune")
+        return best.value

@@ -18,6 +18,8 @@
 #include <vector>
 
 #include "lbm_device.hpp"
+#include "lbm_error.hpp"
+#include "lbm_setup.hpp"
 
 using namespace lbm_dev;
 
@@ -26,11 +28,13 @@ using namespace lbm_dev;
 // ---------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
-static int fail(int code, const std::string &msg)
+int lbm_set_error(int code, const std::string &msg)
 {
     g_last_error = msg;
     return code;
 }
+
+static int fail(int code, const std::string &msg) { return lbm_set_error(code, msg); }
 
 #define LBM_HIP_TRY(expr)                                                                          \
     do                                                                                             \
@@ -285,199 +289,6 @@ __global__ void k_refresh_cols(T *lat, uint32_t Vx, uint32_t P, uint32_t X0, uin
 #pragma unroll
     for (uint32_t d = 0; d < 9; d += 3)
         lat[at<L>(n, d, Sd)] = lat[at<L>(n + 1u, d, Sd)];
-}
-
-// ---------------------------------------------------------------------------
-// set-up kernels
-// ---------------------------------------------------------------------------
-struct Geom
-{
-    uint32_t Hx, N, P, X0, Sd;
-};
-
-__device__ __forceinline__ uint32_t place_logical(uint32_t n, const Geom &g)
-{
-    return place(n % g.Hx, n / g.Hx, g.P, g.X0);
-}
-
-template <int L, int S>
-__global__ void k_init_rest(typename Storage<S>::T *lat, Geom g)
-{
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.N)
-        return;
-    const uint32_t pn = place_logical(n, g);
-#pragma unroll
-    for (int d = 0; d < 9; ++d)
-        lat[at<L>(pn, uint32_t(d), g.Sd)] = Storage<S>::encode(weight<double>(d), d);
-}
-
-struct InoutValues
-{
-    double v[18];
-};
-
-// Inlet ghost column x = 0 and outlet ghost column x = H-1, rows 1..V-2, at
-// their equilibria (domain_initialization.hpp:382-398).
-template <int L, int S>
-__global__ void k_init_inout(typename Storage<S>::T *lat, uint32_t H, uint32_t V, Geom g, uint32_t sub_h,
-                             uint32_t sub_v, int buffered, InoutValues vals)
-{
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t side = blockIdx.y;
-    if (j >= V - 2u)
-        return;
-    const uint32_t x = 1u + side * (H - 3u);
-    const uint32_t y = 1u + j;
-    const uint32_t xe = buffered ? x + (x - 1u) / sub_h : x;
-    const uint32_t ye = buffered ? y + (y - 1u) / sub_v : y;
-    const uint32_t node = place(side ? xe + 1u : xe - 1u, ye, g.P, g.X0);
-#pragma unroll
-    for (int d = 0; d < 9; ++d)
-        lat[at<L>(node, uint32_t(d), g.Sd)] = Storage<S>::encode(vals.v[side * 9 + d], d);
-}
-
-struct CodeParams
-{
-    uint32_t H, V, Hx, N, sub_h, sub_v, P, X0;
-    int buffered;
-    int outlet_active;
-};
-
-__device__ __forceinline__ uint32_t map_axis(uint32_t u, uint32_t last, uint32_t sub, int buffered)
-{
-    // unexpanded coordinate -> expanded (BufferedNodeAccess, access.hpp:353-362);
-    // the high ghost sits right after the last real node
-    if (!buffered)
-        return u;
-    if (u == 0u)
-        return 0u;
-    if (u == last)
-        return (last - 1u) + (last - 2u) / sub + 1u;
-    return u + (u - 1u) / sub;
-}
-
-// Codes of every logical node n, stored at its physical place.  Padding
-// places keep 0 (not fluid, not solid).
-__global__ void k_build_codes(const int8_t *phase, uint16_t *code, CodeParams cp)
-{
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= cp.N)
-        return;
-    const int8_t ph = phase[n];
-    uint32_t c = 0;
-    const uint32_t xe = n % cp.Hx, ye = n / cp.Hx;
-    if (ph == 1)
-        c = LBM_CODE_SOLID;
-    else if (ph == 0)
-    {
-        const uint32_t x = cp.buffered ? xe - xe / (cp.sub_h + 1u) : xe;
-        const uint32_t y = cp.buffered ? ye - ye / (cp.sub_v + 1u) : ye;
-        c = LBM_CODE_FLUID;
-        for (int d = 0; d < 9; ++d)
-        {
-            if (d == 4)
-                continue;
-            const uint32_t sx = uint32_t(int32_t(x) - cx_of(d));
-            const uint32_t sy = uint32_t(int32_t(y) - cy_of(d));
-            const uint32_t m = map_axis(sx, cp.H - 1u, cp.sub_h, cp.buffered) +
-                               map_axis(sy, cp.V - 1u, cp.sub_v, cp.buffered) * cp.Hx;
-            if (phase[m] == 1)
-                c |= 1u << d;
-        }
-        if (cp.outlet_active && x == cp.H - 2u && y >= 2u && y + 3u <= cp.V)
-            c |= LBM_CODE_OUTLET;
-        if (x + 3u == cp.H && y >= 2u && y + 3u <= cp.V)
-            c |= LBM_CODE_UY_SOURCE;
-    }
-    code[place(xe, ye, cp.P, cp.X0)] = uint16_t(c);
-}
-
-__global__ void k_gather_codes(const uint16_t *code, uint16_t *out, Geom g)
-{
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < g.N)
-        out[n] = code[place_logical(n, g)];
-}
-
-__global__ void k_fill(double *p, size_t n, double v)
-{
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n)
-        p[i] = v;
-}
-
-// logical flat index i of a layout (reference order over N nodes) ->
-// (node, direction)
-template <int L>
-__device__ __forceinline__ void split_flat(uint32_t i, uint32_t N, uint32_t &n, uint32_t &d)
-{
-    if constexpr (L == LBM_LAYOUT_COLLISION)
-    {
-        n = i / 9u;
-        d = i % 9u;
-    }
-    else if constexpr (L == LBM_LAYOUT_STREAM)
-    {
-        d = i / N;
-        n = i % N;
-    }
-    else
-    {
-        const uint32_t g = i / (3u * N), r = i % (3u * N);
-        n = r / 3u;
-        d = 3u * g + r % 3u;
-    }
-}
-
-template <int L, int S>
-__global__ void k_decode(const typename Storage<S>::T *lat, double *out, Geom g)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 9u * g.N)
-        return;
-    uint32_t n, d;
-    split_flat<L>(i, g.N, n, d);
-    out[i] = Storage<S>::decode(lat[at<L>(place_logical(n, g), d, g.Sd)], int(d));
-}
-
-template <int L, int S>
-__global__ void k_encode(typename Storage<S>::T *lat, const double *in, Geom g)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 9u * g.N)
-        return;
-    uint32_t n, d;
-    split_flat<L>(i, g.N, n, d);
-    lat[at<L>(place_logical(n, g), d, g.Sd)] = Storage<S>::encode(in[i], int(d));
-}
-
-// Equilibrium of a given (rho, j) field on every fluid node (warm start).
-template <int L, int S>
-__global__ void k_set_equilibrium(typename Storage<S>::T *lat, const uint16_t *code, const double *rho,
-                                  const double *jx, const double *jy, Geom g, uint32_t H, uint32_t sub_h,
-                                  uint32_t sub_v, int buffered)
-{
-    using C = typename Storage<S>::C;
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.N)
-        return;
-    const uint32_t pn = place_logical(n, g);
-    if (!(code[pn] & LBM_CODE_FLUID))
-        return;
-    const uint32_t xe = n % g.Hx, ye = n / g.Hx;
-    const uint32_t x = buffered ? xe - xe / (sub_h + 1u) : xe;
-    const uint32_t y = buffered ? ye - ye / (sub_v + 1u) : ye;
-    const uint32_t r = (x - 1u) + (y - 1u) * (H - 2u);
-    const C rh = C(rho[r]), ux = C(jx[r]), uy = C(jy[r]);
-    const C usq = ux * ux + uy * uy;
-#pragma unroll
-    for (int d = 0; d < 9; ++d)
-    {
-        const C cu = C(cx_of(d)) * ux + C(cy_of(d)) * uy;
-        const C feq = weight<C>(d) * (rh + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
-        Storage<S>::store(lat, at<L>(pn, uint32_t(d), g.Sd), feq, d);
-    }
 }
 
 // ---------------------------------------------------------------------------

@@ -29,6 +29,7 @@ bool Extensions::is_default() const
     const Extensions d;
     return storage == d.storage && nan_protection == d.nan_protection && seed == d.seed && porosity == d.porosity &&
            radius_min == d.radius_min && radius_max == d.radius_max && obstacle_margin == d.obstacle_margin &&
+           max_attempts == d.max_attempts &&
            cylinder_x == d.cylinder_x && cylinder_radius == d.cylinder_radius;
 }
 
@@ -443,7 +444,7 @@ std::vector<Disc> porous_discs(unsigned int H, unsigned int V, const Extensions 
     std::vector<std::vector<std::uint32_t>> grid(gw * gh);
 
     double solid = 0.0;
-    for (std::uint64_t attempt = 0; attempt < (1ull << 24) && solid < target; ++attempt)
+    for (std::uint64_t attempt = 0; attempt < ext.max_attempts && solid < target; ++attempt)
     {
         const double r = ext.radius_min + (ext.radius_max - ext.radius_min) * rng.uniform();
         const double x = x_lo + std::floor(rng.uniform() * (x_hi - x_lo + 1.0));

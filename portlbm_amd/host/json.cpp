@@ -535,6 +535,7 @@ JsonValue properties_to_json_value(const core::Properties &p)
         x["radiusMin"] = JsonValue::number(e.radius_min);
         x["radiusMax"] = JsonValue::number(e.radius_max);
         x["obstacleMargin"] = JsonValue::unsigned_number(e.obstacle_margin);
+        x["maxAttempts"] = JsonValue::unsigned_number(e.max_attempts);
         x["cylinderX"] = JsonValue::unsigned_number(e.cylinder_x);
         x["cylinderRadius"] = JsonValue::number(e.cylinder_radius);
     }
@@ -589,6 +590,8 @@ core::Properties json_value_to_properties(const JsonValue &data, int offset)
             e.radius_max = f->as_double("mi355x.radiusMax");
         if (const JsonValue *f = x->find("obstacleMargin"))
             e.obstacle_margin = read_uint(*f, "mi355x.obstacleMargin");
+        if (const JsonValue *f = x->find("maxAttempts"))
+            e.max_attempts = f->as_unsigned("mi355x.maxAttempts");
         if (const JsonValue *f = x->find("cylinderX"))
             e.cylinder_x = read_uint(*f, "mi355x.cylinderX");
         if (const JsonValue *f = x->find("cylinderRadius"))
