@@ -239,6 +239,14 @@ int lbm_batch_get_tiles(lbm_batch *b, uint32_t tile_width[4]);
  * class (coordinate descent, `steps_per_trial` steps each), keeps the fastest
  * and re-initialises the domains (trials advance the state). */
 int lbm_batch_autotune(lbm_batch *b, uint32_t steps_per_trial, double *best_ms_per_step);
+/* Per-domain steady-state residual ||j^now - j^snap||_2 / ||j^now||_2 over
+ * fluid nodes (FP64), then j^snap := j^now; out holds one value per domain.
+ * The first call after init compares against zero and returns 1. */
+int lbm_batch_residuals(lbm_batch *b, double *out, uint32_t count);
+/* Equilibrium of a (density, momentum) field on every fluid node of one
+ * domain ((H-2)*(V-2) values each, inner row-major) -- the warm start. */
+int lbm_batch_set_equilibrium(lbm_batch *b, uint32_t domain, const double *density, const double *x_velocity,
+                              const double *y_velocity, size_t count);
 
 #ifdef __cplusplus
 }

@@ -533,6 +533,8 @@ def _batch_sigs() -> ctypes.CDLL:
             "lbm_batch_set_tiles": ([_P, _P], ctypes.c_int),
             "lbm_batch_get_tiles": ([_P, _P], ctypes.c_int),
             "lbm_batch_autotune": ([_P, _U32, _DP], ctypes.c_int),
+            "lbm_batch_residuals": ([_P, _P, _U32], ctypes.c_int),
+            "lbm_batch_set_equilibrium": ([_P, _U32, _P, _P, _P, _SZ], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -615,6 +617,18 @@ class Batch:
     def set_tiles(self, widths: list[int]) -> None:
         arr = (ctypes.c_uint32 * 4)(*widths)
         _check_hip(self.lib.lbm_batch_set_tiles(self._b, arr), "lbm_batch_set_tiles")
+
+    def residuals(self) -> np.ndarray:
+        out = np.empty(len(self.shapes))
+        _check_hip(self.lib.lbm_batch_residuals(self._b, _ptr(out), out.size), "lbm_batch_residuals")
+        return out
+
+    def set_equilibrium(self, domain: int, rho: np.ndarray, jx: np.ndarray, jy: np.ndarray) -> None:
+        H, V = self.shapes[domain]
+        D = (H - 2) * (V - 2)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (rho, jx, jy)]
+        _check_hip(self.lib.lbm_batch_set_equilibrium(self._b, domain, *map(_ptr, arrs), D),
+                   "lbm_batch_set_equilibrium")
 
     def autotune(self, steps_per_trial: int = 20) -> float:
         best = ctypes.c_double(0)

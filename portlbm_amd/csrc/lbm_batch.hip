@@ -106,6 +106,63 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_step(BatchArgs a)
     }
 }
 
+// Per-tile FP64 sums of |j - j_snap|^2 and |j|^2 over fluid nodes of the
+// source lattice, snapshot update; one block per tile, wave64 shuffles.
+template <int S>
+__global__ __launch_bounds__(kTileThreads) void k_batch_residual(const BatchDom *doms, const BatchTile *tiles,
+                                                                 const void *src_v, const uint16_t *codes,
+                                                                 double *snap, double *partials)
+{
+    using T = typename Storage<S>::T;
+    const T *src = static_cast<const T *>(src_v);
+    const BatchTile tl = tiles[blockIdx.x];
+    const BatchDom dm = doms[tl.dom];
+    const uint32_t lx = threadIdx.x % dm.tile_w, ly = threadIdx.x / dm.tile_w;
+    const uint32_t x = tl.x0 + lx, y = tl.y0 + ly;
+    double acc0 = 0.0, acc1 = 0.0;
+    if (x + 2u <= dm.H && y + 2u <= dm.V)
+    {
+        const uint32_t pn = kBatchX0 + x + y * dm.P;
+        if (codes[dm.code + pn] & LBM_CODE_FLUID)
+        {
+            double jx = 0, jy = 0;
+#pragma unroll
+            for (int d = 0; d < 9; ++d)
+            {
+                const double f = Storage<S>::decode(src[dm.lat + at<LBM_LAYOUT_STREAM>(pn, uint32_t(d), dm.S)], d);
+                jx += f * double(cx_of(d));
+                jy += f * double(cy_of(d));
+            }
+            double *sp = snap + 2ull * (dm.code + pn);
+            const double dx = jx - sp[0], dy = jy - sp[1];
+            acc0 = dx * dx + dy * dy;
+            acc1 = jx * jx + jy * jy;
+            sp[0] = jx;
+            sp[1] = jy;
+        }
+    }
+    __shared__ double part[kTileThreads / 64][2];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        acc0 += __shfl_xor(acc0, off, 64);
+        acc1 += __shfl_xor(acc1, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        part[threadIdx.x >> 6][0] = acc0;
+        part[threadIdx.x >> 6][1] = acc1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+    {
+        double s = 0.0;
+        for (int w = 0; w < kTileThreads / 64; ++w)
+            s += part[w][threadIdx.x];
+        partials[2ull * blockIdx.x + threadIdx.x] = s;
+    }
+}
+
 int width_class(uint32_t inner_w) { return inner_w <= 128 ? 0 : inner_w <= 256 ? 1 : inner_w <= 512 ? 2 : 3; }
 
 }  // namespace
@@ -139,6 +196,8 @@ struct lbm_batch
     hipGraphExec_t graph = nullptr;
     uint32_t graph_steps = 0;
     int graph_parity = -1;
+    double *snap = nullptr;      // 2 doubles per node place (residual snapshot, lazy)
+    bool snap_valid = false;
 };
 
 namespace
@@ -285,6 +344,7 @@ void lbm_batch_destroy(lbm_batch *b)
     (void)hipFree(b->uy);
     (void)hipFree(b->d_doms);
     (void)hipFree(b->d_tiles);
+    (void)hipFree(b->snap);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -411,6 +471,7 @@ int lbm_batch_init(lbm_batch *b)
     b->fluid_nodes = uint64_t(std::count_if(codes.begin(), codes.end(), [](uint16_t c) { return c & LBM_CODE_FLUID; }));
     b->iteration = 0;
     b->initialized = true;
+    b->snap_valid = false;
     return LBM_OK;
 }
 
@@ -565,6 +626,117 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps_per_trial, double *best_ms_p
     if (best_ms_per_step)
         *best_ms_per_step = best;
     return lbm_batch_init(b);
+}
+
+int lbm_batch_residuals(lbm_batch *b, double *out, uint32_t count)
+{
+    if (!b || !out || count != b->doms.size())
+        return lbm_set_error(LBM_ERR_INVALID_ARGUMENT, "lbm_batch_residuals: count must equal the domain count");
+    if (!b->initialized)
+        return lbm_set_error(LBM_ERR_STATE, "lbm_batch_residuals: call lbm_batch_init first");
+    if (!b->snap)
+        BATCH_TRY(hipMalloc(&b->snap, 2ull * b->code_nodes * sizeof(double)));
+    if (!b->snap_valid)
+    {
+        BATCH_TRY(hipMemsetAsync(b->snap, 0, 2ull * b->code_nodes * sizeof(double), b->stream));
+        b->snap_valid = true;
+    }
+    double *partials = nullptr;
+    BATCH_TRY(hipMalloc(&partials, 2ull * std::max(1u, b->ntiles) * sizeof(double)));
+    const void *src = b->arena[b->iteration & 1u];
+    switch (b->storage)
+    {
+    case LBM_STORE_F64:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_F64>), dim3(b->ntiles), dim3(kTileThreads), 0, b->stream,
+                           b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    case LBM_STORE_F32:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_F32>), dim3(b->ntiles), dim3(kTileThreads), 0, b->stream,
+                           b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    case LBM_STORE_F16_SHIFTED:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_F16_SHIFTED>), dim3(b->ntiles), dim3(kTileThreads), 0,
+                           b->stream, b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    default:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_BF16_SHIFTED>), dim3(b->ntiles), dim3(kTileThreads), 0,
+                           b->stream, b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    }
+    hipError_t e = hipGetLastError();
+    std::vector<double> host(2ull * b->ntiles);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host.data(), partials, host.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(b->stream);
+    (void)hipFree(partials);
+    if (e != hipSuccess)
+        return lbm_set_error(LBM_ERR_DEVICE, std::string("lbm_batch_residuals: ") + hipGetErrorString(e));
+    for (size_t d = 0; d < b->doms.size(); ++d)
+    {
+        double num = 0.0, den = 0.0;
+        for (uint32_t t = b->tile_first[d]; t < b->tile_first[d] + b->tile_count[d]; ++t)
+        {
+            num += host[2ull * t];
+            den += host[2ull * t + 1];
+        }
+        out[d] = den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? 1.0 : 0.0);
+    }
+    return LBM_OK;
+}
+
+int lbm_batch_set_equilibrium(lbm_batch *b, uint32_t domain, const double *rho, const double *jx, const double *jy,
+                              size_t count)
+{
+    if (!b || !rho || !jx || !jy || domain >= b->doms.size())
+        return lbm_set_error(LBM_ERR_INVALID_ARGUMENT, "lbm_batch_set_equilibrium: bad argument");
+    if (!b->initialized)
+        return lbm_set_error(LBM_ERR_STATE, "lbm_batch_set_equilibrium: call lbm_batch_init first");
+    const BatchDom &dm = b->doms[domain];
+    if (count != dm.D)
+        return lbm_set_error(LBM_ERR_INVALID_ARGUMENT, "lbm_batch_set_equilibrium: count must be (H-2)*(V-2)");
+    double *tmp = nullptr;
+    BATCH_TRY(hipMalloc(&tmp, 3 * count * sizeof(double)));
+    hipError_t e = hipSuccess;
+    const double *src[3] = {rho, jx, jy};
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+        e = hipMemcpyAsync(tmp + k * count, src[k], count * sizeof(double), hipMemcpyHostToDevice, b->stream);
+    const Geom g{dm.H, dm.H * dm.V, dm.P, kBatchX0, dm.S};
+    void *lat = b->arena[b->iteration & 1u];
+    if (e == hipSuccess)
+    {
+        const dim3 grid((g.N + 255u) / 256u), block(256);
+        switch (b->storage)
+        {
+        case LBM_STORE_F64:
+            hipLaunchKernelGGL((k_set_equilibrium<LBM_LAYOUT_STREAM, LBM_STORE_F64>), grid, block, 0, b->stream,
+                               static_cast<double *>(lat) + dm.lat, b->codes + dm.code, tmp, tmp + count,
+                               tmp + 2 * count, g, dm.H, dm.H, dm.V, 0);
+            break;
+        case LBM_STORE_F32:
+            hipLaunchKernelGGL((k_set_equilibrium<LBM_LAYOUT_STREAM, LBM_STORE_F32>), grid, block, 0, b->stream,
+                               static_cast<float *>(lat) + dm.lat, b->codes + dm.code, tmp, tmp + count,
+                               tmp + 2 * count, g, dm.H, dm.H, dm.V, 0);
+            break;
+        case LBM_STORE_F16_SHIFTED:
+            hipLaunchKernelGGL((k_set_equilibrium<LBM_LAYOUT_STREAM, LBM_STORE_F16_SHIFTED>), grid, block, 0,
+                               b->stream, static_cast<_Float16 *>(lat) + dm.lat, b->codes + dm.code, tmp, tmp + count,
+                               tmp + 2 * count, g, dm.H, dm.H, dm.V, 0);
+            break;
+        default:
+            hipLaunchKernelGGL((k_set_equilibrium<LBM_LAYOUT_STREAM, LBM_STORE_BF16_SHIFTED>), grid, block, 0,
+                               b->stream, static_cast<uint16_t *>(lat) + dm.lat, b->codes + dm.code, tmp, tmp + count,
+                               tmp + 2 * count, g, dm.H, dm.H, dm.V, 0);
+            break;
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return lbm_set_error(LBM_ERR_DEVICE, std::string("lbm_batch_set_equilibrium: ") + hipGetErrorString(e));
+    return LBM_OK;
 }
 
 }  // extern "C"

@@ -2,6 +2,8 @@
 domain run alone -- against the oracle in FP64, bit-exactly against the
 single-lattice library in FP32 -- in all three launch modes, across repeated
 graph replays and after autotuning."""
+import json
+
 import numpy as np
 import pytest
 
@@ -80,3 +82,51 @@ def test_set_tiles_rejects_bad_widths():
             b.set_tiles([64, 96, 128, 256])
         b.set_tiles([64, 64, 128, 256])
         assert b.tiles() == [64, 64, 128, 256]
+
+
+def _fluid_mask(doc):
+    lin = json.loads(json.dumps(doc))
+    lin["algorithmic"]["algorithm"] = "lptl"
+    phase = pa.build_phase(lin)
+    H, V = doc["domain"]["horizontalNodes"] + 2, doc["domain"]["verticalNodes"] + 2
+    return (phase.reshape(V, H)[1:-1, 1:-1] == 0).ravel()
+
+
+def test_residuals_match_host_momentum_difference():
+    docs = _small_batch(count=6, seed=11, storage="f64")
+    with pa.Batch(docs, storage="f64") as b:
+        b.run(120, "persistent")
+        first = b.residuals()
+        assert np.all(first == 1.0)  # compared against the zero snapshot
+        m1 = [b.macros(i) for i in range(len(docs))]
+        b.run(40, "persistent")
+        r = b.residuals()
+        m2 = [b.macros(i) for i in range(len(docs))]
+        again = b.residuals()  # no step in between: the snapshot is current
+        assert np.all(again == 0.0)
+    for i, doc in enumerate(docs):
+        fl = _fluid_mask(doc)
+        dx, dy = m2[i][1][fl] - m1[i][1][fl], m2[i][2][fl] - m1[i][2][fl]
+        host = np.sqrt(np.sum(dx * dx + dy * dy) / np.sum(m2[i][1][fl] ** 2 + m2[i][2][fl] ** 2))
+        assert r[i] > 0 and abs(r[i] - host) <= 1e-6 * host, (i, r[i], host)
+
+
+def test_batch_equilibrium_warm_start_equals_single_lattice():
+    docs = _small_batch(count=4, seed=13, storage="f32")
+    with pa.Batch(docs, storage="f32") as b:
+        for i, doc in enumerate(docs):
+            H, V = b.shapes[i]
+            yy, xx = np.mgrid[0:V - 2, 0:H - 2]
+            rho = 1.0 + 0.01 * np.sin(xx / 7.0) * np.cos(yy / 5.0)
+            jx = 0.02 * np.cos(yy / 3.0)
+            jy = 0.005 * np.sin(xx / 4.0)
+            b.set_equilibrium(i, rho, jx, jy)
+            with pa.Lattice.from_settings(doc) as lat:
+                lat.set_equilibrium_field(rho, jx, jy)
+                lat.run(50)
+                expect = lat.macros()
+            b.run(50, "persistent")  # advances every domain; only domain i is compared at this point
+            got = b.macros(i)
+            b.init()
+            for a, e in zip(got, expect):
+                assert np.array_equal(a, e), i
