@@ -241,8 +241,12 @@ int lbm_batch_get_tiles(lbm_batch *b, uint32_t tile_width[4]);
 int lbm_batch_autotune(lbm_batch *b, uint32_t steps_per_trial, double *best_ms_per_step);
 /* Per-domain steady-state residual ||j^now - j^snap||_2 / ||j^now||_2 over
  * fluid nodes (FP64), then j^snap := j^now; out holds one value per domain.
- * The first call after init compares against zero and returns 1. */
+ * The first call after init compares against zero and returns 1; a domain
+ * whose populations hold NaN or Inf yields NaN. */
 int lbm_batch_residuals(lbm_batch *b, double *out, uint32_t count);
+/* Domain `domain` back to the at-rest start of lbm_batch_init (populations,
+ * inlet/outlet ghosts, outlet history); the other domains are untouched. */
+int lbm_batch_reset_domain(lbm_batch *b, uint32_t domain);
 /* Equilibrium of a (density, momentum) field on every fluid node of one
  * domain ((H-2)*(V-2) values each, inner row-major) -- the warm start. */
 int lbm_batch_set_equilibrium(lbm_batch *b, uint32_t domain, const double *density, const double *x_velocity,

@@ -534,6 +534,7 @@ def _batch_sigs() -> ctypes.CDLL:
             "lbm_batch_get_tiles": ([_P, _P], ctypes.c_int),
             "lbm_batch_autotune": ([_P, _U32, _DP], ctypes.c_int),
             "lbm_batch_residuals": ([_P, _P, _U32], ctypes.c_int),
+            "lbm_batch_reset_domain": ([_P, _U32], ctypes.c_int),
             "lbm_batch_set_equilibrium": ([_P, _U32, _P, _P, _P, _SZ], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
@@ -622,6 +623,9 @@ class Batch:
         out = np.empty(len(self.shapes))
         _check_hip(self.lib.lbm_batch_residuals(self._b, _ptr(out), out.size), "lbm_batch_residuals")
         return out
+
+    def reset_domain(self, domain: int) -> None:
+        _check_hip(self.lib.lbm_batch_reset_domain(self._b, domain), "lbm_batch_reset_domain")
 
     def set_equilibrium(self, domain: int, rho: np.ndarray, jx: np.ndarray, jy: np.ndarray) -> None:
         H, V = self.shapes[domain]

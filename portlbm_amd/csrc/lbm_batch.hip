@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -279,6 +280,36 @@ int run_persistent_steps(lbm_batch *b, uint32_t steps, uint64_t first_iteration)
     return LBM_OK;
 }
 
+// Rest populations and inlet/outlet ghost values of domain d in lattice l0.
+template <int S>
+int init_domain_lattice(lbm_batch *b, size_t d, typename Storage<S>::T *l0)
+{
+    const BatchDom &dm = b->doms[d];
+    const lbm_batch_domain &ds = b->desc[d];
+    const Geom g{dm.H, dm.H * dm.V, dm.P, kBatchX0, dm.S};
+    hipLaunchKernelGGL((k_init_rest<LBM_LAYOUT_STREAM, S>), dim3((g.N + 255u) / 256u), dim3(256), 0, b->stream,
+                       l0 + dm.lat, g);
+    InoutValues v;
+    static const double w[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9,
+                                1.0 / 9,  1.0 / 36, 1.0 / 9, 1.0 / 36};
+    for (int k = 0; k < 9; ++k)
+    {
+        const int cx = k % 3 - 1, cy = k / 3 - 1;
+        const double ui = cx * ds.inlet_velocity_x + cy * ds.inlet_velocity_y;
+        const double uo = cx * ds.outlet_velocity_x + cy * ds.outlet_velocity_y;
+        v.v[k] = w[k] * (ds.inlet_density + 3 * ui + 9.0 / 2 * std::pow(ui, 2) -
+                         3.0 / 2 * (ds.inlet_velocity_x * ds.inlet_velocity_x +
+                                    ds.inlet_velocity_y * ds.inlet_velocity_y));
+        v.v[9 + k] = w[k] * (ds.outlet_density + 3 * uo + 9.0 / 2 * std::pow(uo, 2) -
+                             3.0 / 2 * (ds.outlet_velocity_x * ds.outlet_velocity_x +
+                                        ds.outlet_velocity_y * ds.outlet_velocity_y));
+    }
+    hipLaunchKernelGGL((k_init_inout<LBM_LAYOUT_STREAM, S>), dim3((dm.V - 2u + 255u) / 256u, 2), dim3(256), 0,
+                       b->stream, l0 + dm.lat, dm.H, dm.V, g, dm.H, dm.V, 0, v);
+    BATCH_TRY(hipGetLastError());
+    return LBM_OK;
+}
+
 template <int S>
 int init_domains(lbm_batch *b)
 {
@@ -291,27 +322,8 @@ int init_domains(lbm_batch *b)
     for (size_t d = 0; d < b->doms.size(); ++d)
     {
         const BatchDom &dm = b->doms[d];
-        const lbm_batch_domain &ds = b->desc[d];
-        const Geom g{dm.H, dm.H * dm.V, dm.P, kBatchX0, dm.S};
-        hipLaunchKernelGGL((k_init_rest<LBM_LAYOUT_STREAM, S>), dim3((g.N + 255u) / 256u), dim3(256), 0, b->stream,
-                           l0 + dm.lat, g);
-        InoutValues v;
-        static const double w[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9,
-                                    1.0 / 9,  1.0 / 36, 1.0 / 9, 1.0 / 36};
-        for (int k = 0; k < 9; ++k)
-        {
-            const int cx = k % 3 - 1, cy = k / 3 - 1;
-            const double ui = cx * ds.inlet_velocity_x + cy * ds.inlet_velocity_y;
-            const double uo = cx * ds.outlet_velocity_x + cy * ds.outlet_velocity_y;
-            v.v[k] = w[k] * (ds.inlet_density + 3 * ui + 9.0 / 2 * std::pow(ui, 2) -
-                             3.0 / 2 * (ds.inlet_velocity_x * ds.inlet_velocity_x +
-                                        ds.inlet_velocity_y * ds.inlet_velocity_y));
-            v.v[9 + k] = w[k] * (ds.outlet_density + 3 * uo + 9.0 / 2 * std::pow(uo, 2) -
-                                 3.0 / 2 * (ds.outlet_velocity_x * ds.outlet_velocity_x +
-                                            ds.outlet_velocity_y * ds.outlet_velocity_y));
-        }
-        hipLaunchKernelGGL((k_init_inout<LBM_LAYOUT_STREAM, S>), dim3((dm.V - 2u + 255u) / 256u, 2), dim3(256), 0,
-                           b->stream, l0 + dm.lat, dm.H, dm.V, g, dm.H, dm.V, 0, v);
+        if (int rc = init_domain_lattice<S>(b, d, l0))
+            return rc;
         const CodeParams cp{dm.H, dm.V, dm.H, dm.H * dm.V, dm.H, dm.V, dm.P, kBatchX0, 0, 1};
         hipLaunchKernelGGL(k_build_codes, dim3((cp.N + 255u) / 256u), dim3(256), 0, b->stream,
                            b->phase + b->phase_off[d], b->codes + dm.code, cp);
@@ -680,8 +692,36 @@ int lbm_batch_residuals(lbm_batch *b, double *out, uint32_t count)
             num += host[2ull * t];
             den += host[2ull * t + 1];
         }
-        out[d] = den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? 1.0 : 0.0);
+        if (!std::isfinite(num) || !std::isfinite(den))
+            out[d] = std::numeric_limits<double>::quiet_NaN();  // a diverged domain never counts as converged
+        else
+            out[d] = den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? 1.0 : 0.0);
     }
+    return LBM_OK;
+}
+
+int lbm_batch_reset_domain(lbm_batch *b, uint32_t domain)
+{
+    if (!b || domain >= b->doms.size())
+        return lbm_set_error(LBM_ERR_INVALID_ARGUMENT, "lbm_batch_reset_domain: bad domain");
+    if (!b->initialized)
+        return lbm_set_error(LBM_ERR_STATE, "lbm_batch_reset_domain: call lbm_batch_init first");
+    const BatchDom &dm = b->doms[domain];
+    void *lat = b->arena[b->iteration & 1u];
+    int rc = LBM_OK;
+    switch (b->storage)
+    {
+    case LBM_STORE_F64: rc = init_domain_lattice<LBM_STORE_F64>(b, domain, static_cast<double *>(lat)); break;
+    case LBM_STORE_F32: rc = init_domain_lattice<LBM_STORE_F32>(b, domain, static_cast<float *>(lat)); break;
+    case LBM_STORE_F16_SHIFTED:
+        rc = init_domain_lattice<LBM_STORE_F16_SHIFTED>(b, domain, static_cast<_Float16 *>(lat));
+        break;
+    default: rc = init_domain_lattice<LBM_STORE_BF16_SHIFTED>(b, domain, static_cast<uint16_t *>(lat)); break;
+    }
+    if (rc)
+        return rc;
+    BATCH_TRY(hipMemsetAsync(b->uy + dm.uy, 0, 2ull * dm.V * sizeof(double), b->stream));
+    BATCH_TRY(hipStreamSynchronize(b->stream));
     return LBM_OK;
 }
 

@@ -1,0 +1,466 @@
+"""Learned warm start for the steady solve (include/lbm_surrogate.h).
+
+A convolutional encoder-decoder, trained on the GPU by this package's own
+HIP kernels on steady states that the batched solver produces, maps an
+obstacle mask, the boundary densities and tau to a steady (density,
+momentum) field.  The field only warm-starts the lattice from its
+equilibrium (lbm_batch_set_equilibrium); the plain solve decides
+convergence, and a domain whose residual fails to fall is reset to the
+at-rest start (lbm_batch_reset_domain).  The reference solver always starts
+from rest (include/lbm/core/domain_initialization.hpp:311-403).
+
+Encoding on an inner grid of ny x nx nodes (the network's H x W):
+  inputs   0: fluid mask (1 fluid, 0 solid)
+           1: x ramp, 2 (x + 0.5) / nx - 1
+           2: (tau - 0.725) / 0.175
+           3: (drho - 0.03) / 0.02, drho = rho_in - rho_out
+  targets  0: (rho - rho_out) / drho
+           1, 2: j (tau - 1/2) / drho * j_scale
+The momentum normalisation is Stokes scaling: at low Reynolds number the
+steady momentum is proportional to the pressure drop over the viscosity,
+nu = (tau - 1/2) / 3, so the network mostly learns the geometry's share.
+j_scale (one number per model) brings the normalised momentum to unit RMS
+over the training set.
+
+    python -m portlbm_amd.surrogate train --samples 20000 --out model.npz --report report.json
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import Batch, LbmError, _check_hip, _ptr, build_phase, hip_library, settings
+
+INPUTS, OUTPUTS = 4, 3
+TAU_MID, TAU_HALF = 0.725, 0.175
+DRHO_MID, DRHO_HALF = 0.03, 0.02
+DEFAULT_MODEL = Path(__file__).resolve().parent / "data" / "surrogate_porous_256.npz"
+
+_P, _U32, _SZ = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_size_t
+_sig_done = False
+
+
+def _lib() -> ctypes.CDLL:
+    global _sig_done
+    lib = hip_library()
+    if not _sig_done:
+        sig = {
+            "lbm_surrogate_param_count": ([], _SZ),
+            "lbm_surrogate_create": ([_U32, _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+            "lbm_surrogate_destroy": ([_P], None),
+            "lbm_surrogate_init_params": ([_P, ctypes.c_uint64], ctypes.c_int),
+            "lbm_surrogate_set_params": ([_P, _P, _SZ], ctypes.c_int),
+            "lbm_surrogate_get_params": ([_P, _P, _SZ], ctypes.c_int),
+            "lbm_surrogate_predict": ([_P, _P, _U32, _P], ctypes.c_int),
+            "lbm_surrogate_loss_grad": ([_P, _P, _P, _U32, ctypes.POINTER(ctypes.c_double), _P, _SZ], ctypes.c_int),
+            "lbm_surrogate_train_step": ([_P, _P, _P, _U32, ctypes.c_float, ctypes.POINTER(ctypes.c_double)],
+                                         ctypes.c_int),
+            "lbm_surrogate_dataset_reserve": ([_P, _SZ], ctypes.c_int),
+            "lbm_surrogate_dataset_add": ([_P, _P, _P, _U32], ctypes.c_int),
+            "lbm_surrogate_dataset_size": ([_P], _SZ),
+            "lbm_surrogate_train": ([_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _P],
+                                    ctypes.c_int),
+            "lbm_surrogate_last_ms": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _sig_done = True
+    return lib
+
+
+def param_count() -> int:
+    return int(_lib().lbm_surrogate_param_count())
+
+
+def _f32(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Surrogate:
+    """Network, optimiser state and device-resident training set on one GPU."""
+
+    def __init__(self, height: int, width: int, max_batch: int = 16):
+        self.lib = _lib()
+        self.height, self.width, self.max_batch = height, width, max_batch
+        self.n = param_count()
+        self._s = ctypes.c_void_p()
+        _check_hip(self.lib.lbm_surrogate_create(height, width, max_batch, ctypes.byref(self._s)),
+                   "lbm_surrogate_create")
+
+    def close(self) -> None:
+        if self._s:
+            self.lib.lbm_surrogate_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_params(self, seed: int) -> None:
+        _check_hip(self.lib.lbm_surrogate_init_params(self._s, seed), "lbm_surrogate_init_params")
+
+    def set_params(self, params: np.ndarray) -> None:
+        p = _f32(params).ravel()
+        _check_hip(self.lib.lbm_surrogate_set_params(self._s, _ptr(p), p.size), "lbm_surrogate_set_params")
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.n, dtype=np.float32)
+        _check_hip(self.lib.lbm_surrogate_get_params(self._s, _ptr(p), p.size), "lbm_surrogate_get_params")
+        return p
+
+    def _check_shape(self, a: np.ndarray, channels: int) -> None:
+        if a.ndim != 4 or a.shape[1:] != (channels, self.height, self.width):
+            raise ValueError(f"expected (B, {channels}, {self.height}, {self.width}), got {a.shape}")
+
+    def predict(self, inputs: np.ndarray) -> np.ndarray:
+        x = _f32(inputs)
+        self._check_shape(x, INPUTS)
+        out = np.empty((x.shape[0], OUTPUTS, self.height, self.width), dtype=np.float32)
+        _check_hip(self.lib.lbm_surrogate_predict(self._s, _ptr(x), x.shape[0], _ptr(out)), "lbm_surrogate_predict")
+        return out
+
+    def loss_grad(self, inputs: np.ndarray, targets: np.ndarray) -> tuple[float, np.ndarray]:
+        x, t = _f32(inputs), _f32(targets)
+        self._check_shape(x, INPUTS)
+        self._check_shape(t, OUTPUTS)
+        loss = ctypes.c_double()
+        g = np.empty(self.n, dtype=np.float32)
+        _check_hip(self.lib.lbm_surrogate_loss_grad(self._s, _ptr(x), _ptr(t), x.shape[0], ctypes.byref(loss),
+                                                    _ptr(g), g.size), "lbm_surrogate_loss_grad")
+        return loss.value, g
+
+    def train_step(self, inputs: np.ndarray, targets: np.ndarray, lr: float) -> float:
+        x, t = _f32(inputs), _f32(targets)
+        self._check_shape(x, INPUTS)
+        self._check_shape(t, OUTPUTS)
+        loss = ctypes.c_double()
+        _check_hip(self.lib.lbm_surrogate_train_step(self._s, _ptr(x), _ptr(t), x.shape[0], lr, ctypes.byref(loss)),
+                   "lbm_surrogate_train_step")
+        return loss.value
+
+    def dataset_reserve(self, capacity: int) -> None:
+        _check_hip(self.lib.lbm_surrogate_dataset_reserve(self._s, capacity), "lbm_surrogate_dataset_reserve")
+
+    def dataset_add(self, inputs: np.ndarray, targets: np.ndarray) -> None:
+        x, t = _f32(inputs), _f32(targets)
+        self._check_shape(x, INPUTS)
+        self._check_shape(t, OUTPUTS)
+        _check_hip(self.lib.lbm_surrogate_dataset_add(self._s, _ptr(x), _ptr(t), x.shape[0]),
+                   "lbm_surrogate_dataset_add")
+
+    def dataset_size(self) -> int:
+        return int(self.lib.lbm_surrogate_dataset_size(self._s))
+
+    def train(self, steps: int, schedule_steps: int, batch: int, lr_max: float, lr_min: float,
+              seed: int = 1) -> np.ndarray:
+        hist = np.empty(steps, dtype=np.float32)
+        _check_hip(self.lib.lbm_surrogate_train(self._s, steps, schedule_steps, batch, lr_max, lr_min, seed,
+                                                _ptr(hist)), "lbm_surrogate_train")
+        return hist
+
+    def last_ms(self) -> float:
+        ms = ctypes.c_double()
+        _check_hip(self.lib.lbm_surrogate_last_ms(self._s, ctypes.byref(ms)), "lbm_surrogate_last_ms")
+        return ms.value
+
+
+# ---------------------------------------------------------------------------
+# encoding
+# ---------------------------------------------------------------------------
+def _physical(doc: dict) -> tuple[float, float, float]:
+    phy = doc["physical"]
+    return phy["relaxationTime"], phy["inletDensity"], phy["outletDensity"]
+
+
+def fluid_mask(doc: dict) -> np.ndarray:
+    """Inner fluid mask (ny x nx, bool) of a settings document's unexpanded lattice."""
+    lin = json.loads(json.dumps(doc))
+    lin["algorithmic"]["algorithm"] = "lptl"
+    H, V = doc["domain"]["horizontalNodes"] + 2, doc["domain"]["verticalNodes"] + 2
+    return build_phase(lin).reshape(V, H)[1:-1, 1:-1] == 0
+
+
+def features(mask: np.ndarray, tau: float, rho_in: float, rho_out: float) -> np.ndarray:
+    ny, nx = mask.shape
+    x = np.empty((INPUTS, ny, nx), dtype=np.float32)
+    x[0] = mask
+    x[1] = ((np.arange(nx) + 0.5) / nx * 2.0 - 1.0)[None, :]
+    x[2] = (tau - TAU_MID) / TAU_HALF
+    x[3] = ((rho_in - rho_out) - DRHO_MID) / DRHO_HALF
+    return x
+
+
+def encode_targets(rho: np.ndarray, jx: np.ndarray, jy: np.ndarray, mask: np.ndarray, tau: float, rho_in: float,
+                   rho_out: float, j_scale: float) -> np.ndarray:
+    ny, nx = mask.shape
+    drho = rho_in - rho_out
+    t = np.zeros((OUTPUTS, ny, nx), dtype=np.float32)
+    s = (tau - 0.5) / drho * j_scale
+    t[0] = np.where(mask, (rho.reshape(ny, nx) - rho_out) / drho, 0.0)
+    t[1] = np.where(mask, jx.reshape(ny, nx) * s, 0.0)
+    t[2] = np.where(mask, jy.reshape(ny, nx) * s, 0.0)
+    return t
+
+
+def decode(out: np.ndarray, mask: np.ndarray, tau: float, rho_in: float, rho_out: float,
+           j_scale: float) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(density, x momentum, y momentum) on the inner grid, row-major; solid
+    nodes get the outlet density at rest (they are not written by the warm start)."""
+    drho = rho_in - rho_out
+    s = drho / ((tau - 0.5) * j_scale)
+    o = out.astype(np.float64)
+    rho = np.where(mask, rho_out + drho * o[0], rho_out)
+    jx = np.where(mask, o[1] * s, 0.0)
+    jy = np.where(mask, o[2] * s, 0.0)
+    return rho.ravel(), jx.ravel(), jy.ravel()
+
+
+# ---------------------------------------------------------------------------
+# model files (numpy .npz, no pickled objects)
+# ---------------------------------------------------------------------------
+class Model:
+    def __init__(self, params: np.ndarray, j_scale: float, height: int, width: int, meta: dict | None = None):
+        self.params = _f32(params)
+        self.j_scale = float(j_scale)
+        self.height, self.width = int(height), int(width)
+        self.meta = meta or {}
+
+    def save(self, path: str | Path) -> None:
+        np.savez(path, params=self.params, j_scale=np.float64(self.j_scale),
+                 shape=np.array([self.height, self.width], dtype=np.int64),
+                 meta=np.array(json.dumps(self.meta)))
+
+    @classmethod
+    def load(cls, path: str | Path = DEFAULT_MODEL) -> "Model":
+        with np.load(path, allow_pickle=False) as z:
+            h, w = (int(v) for v in z["shape"])
+            return cls(z["params"], float(z["j_scale"]), h, w, json.loads(str(z["meta"])))
+
+    def network(self, max_batch: int = 16) -> Surrogate:
+        net = Surrogate(self.height, self.width, max_batch)
+        net.set_params(self.params)
+        return net
+
+
+# ---------------------------------------------------------------------------
+# steady states from the batched solver
+# ---------------------------------------------------------------------------
+def porous_docs(count: int, seed: int, size: int = 256, storage: str = "f64") -> list[dict]:
+    """Porous channels of size x size inner nodes: seeded circles (radii
+    U[2, 8), obstacle margin 8), porosity U[0.6, 0.9], tau U[0.55, 0.9],
+    inlet density U[1.01, 1.05], outlet 1.0."""
+    rng = np.random.default_rng(seed)
+    docs = []
+    for _ in range(count):
+        docs.append(settings("lptl", "stream", inner_h=size, inner_v=size, scenario="porous-circles",
+                             work_group_size=256, time_steps=1, tau=float(rng.uniform(0.55, 0.9)),
+                             rho_in=float(rng.uniform(1.01, 1.05)), rho_out=1.0, storage=storage,
+                             seed=int(rng.integers(1, 2**62)), porosity=float(rng.uniform(0.6, 0.9)),
+                             radiusMin=2.0, radiusMax=8.0, obstacleMargin=8, maxAttempts=20000))
+    return docs
+
+
+def solve_steady(docs: list[dict], steps: int, storage: str = "f32", final_check: int = 100):
+    """Runs every domain `steps` steps in one batch; returns the macros of
+    each domain and its residual over the last `final_check` steps."""
+    with Batch(docs, storage=storage) as b:
+        b.run(steps - final_check, "graph")
+        b.residuals()
+        b.run(final_check, "graph")
+        r = b.residuals()
+        macros = [b.macros(i) for i in range(len(docs))]
+    return macros, r
+
+
+def generate(net: Surrogate, docs: list[dict], steps: int, chunk: int, j_scale: float | None,
+             max_residual: float = 1e-3, log=print) -> dict:
+    """Adds the steady states of `docs` to the network's device dataset.
+    Domains that diverge or whose residual stays above `max_residual` are
+    dropped.  j_scale None: taken from the first chunk (unit RMS)."""
+    kept = dropped = 0
+    t0 = time.time()
+    for c0 in range(0, len(docs), chunk):
+        part = docs[c0:c0 + chunk]
+        macros, r = solve_steady(part, steps)
+        xs, raw = [], []
+        for doc, (rho, jx, jy, _), ri in zip(part, macros, r):
+            if not np.isfinite(ri) or ri > max_residual or not np.all(np.isfinite(rho)):
+                dropped += 1
+                continue
+            tau, rin, rout = _physical(doc)
+            mask = fluid_mask(doc)
+            xs.append(features(mask, tau, rin, rout))
+            raw.append((rho, jx, jy, mask, tau, rin, rout))
+        if j_scale is None:
+            acc = [np.concatenate([(jx * (tau - 0.5) / (rin - rout))[mask.ravel()],
+                                   (jy * (tau - 0.5) / (rin - rout))[mask.ravel()]])
+                   for rho, jx, jy, mask, tau, rin, rout in raw]
+            j_scale = float(1.0 / np.sqrt(np.mean(np.concatenate(acc) ** 2)))
+        ts = [encode_targets(*rw[:3], rw[3], rw[4], rw[5], rw[6], j_scale) for rw in raw]
+        if xs:
+            net.dataset_add(np.stack(xs), np.stack(ts))
+        kept += len(xs)
+        log(f"dataset: {c0 + len(part)}/{len(docs)} solved, kept {kept}, dropped {dropped}, "
+            f"{time.time() - t0:.1f} s")
+    return {"kept": kept, "dropped": dropped, "j_scale": j_scale, "seconds": time.time() - t0}
+
+
+# ---------------------------------------------------------------------------
+# warm start and steps to steady state
+# ---------------------------------------------------------------------------
+def warm_start(batch: Batch, docs: list[dict], model: Model, net: Surrogate | None = None) -> float:
+    """Sets every domain of `batch` to the equilibrium of the predicted field;
+    returns the inference time (ms, HIP events)."""
+    own = net is None
+    net = net or model.network()
+    try:
+        masks = [fluid_mask(d) for d in docs]
+        x = np.stack([features(m, *_physical(d)) for m, d in zip(masks, docs)])
+        out = net.predict(x)
+        ms = net.last_ms()
+        for i, (d, m) in enumerate(zip(docs, masks)):
+            batch.set_equilibrium(i, *decode(out[i], m, *_physical(d), model.j_scale))
+    finally:
+        if own:
+            net.close()
+    return ms
+
+
+def steps_to_residual(batch: Batch, tol: float = 1e-6, check: int = 100, max_steps: int = 60000,
+                      fallback_after: int | None = None) -> dict:
+    """Steps until ||j^{t+check} - j^t|| / ||j^{t+check}|| < tol per domain
+    (-1 if not reached within max_steps).  With fallback_after = n, a domain
+    whose residual turns NaN, or after n checks has not fallen below half its
+    first value, is reset to rest and solved plainly; its count includes the
+    steps spent before the reset."""
+    n = len(batch.shapes)
+    done = np.full(n, -1, dtype=np.int64)
+    first = np.full(n, np.nan)
+    since = np.zeros(n, dtype=np.int64)   # checks since the (re)start
+    fell_back = np.zeros(n, dtype=bool)
+    batch.residuals()
+    steps = 0
+    while steps < max_steps and np.any(done < 0):
+        batch.run(check, "graph")
+        steps += check
+        r = batch.residuals()
+        since += 1
+        for d in np.nonzero(done < 0)[0]:
+            if fallback_after is not None and not fell_back[d]:
+                if since[d] == 1:
+                    first[d] = r[d]
+                bad = not np.isfinite(r[d]) or (since[d] >= fallback_after and not r[d] < 0.5 * first[d])
+                if bad:
+                    batch.reset_domain(int(d))
+                    fell_back[d] = True
+                    since[d] = 0
+                    continue
+            # after a reset the snapshot still holds the pre-reset field: skip that first check
+            if np.isfinite(r[d]) and r[d] < tol and since[d] >= (2 if fell_back[d] else 1):
+                done[d] = steps
+    return {"steps": done, "fell_back": fell_back}
+
+
+def compare(docs: list[dict], model: Model, tol: float = 1e-6, max_steps: int = 60000, fallback_after: int = 20,
+            log=print) -> dict:
+    """Steps to `tol` from rest and from the warm start, same domains, FP64."""
+    with Batch(docs, storage="f64") as b:
+        t0 = time.time()
+        rest = steps_to_residual(b, tol, max_steps=max_steps)["steps"]
+        log(f"rest start: {time.time() - t0:.1f} s")
+    with Batch(docs, storage="f64") as b:
+        t0 = time.time()
+        infer_ms = warm_start(b, docs, model)
+        res = steps_to_residual(b, tol, max_steps=max_steps, fallback_after=fallback_after)
+        log(f"warm start: {time.time() - t0:.1f} s (inference {infer_ms:.1f} ms)")
+    warm = res["steps"]
+    ok = (rest > 0) & (warm > 0)
+    ratio = rest[ok] / warm[ok]
+    return {
+        "domains": len(docs), "both_converged": int(ok.sum()), "rest_unconverged": int(np.sum(rest < 0)),
+        "warm_unconverged": int(np.sum(warm < 0)), "fallbacks": int(res["fell_back"].sum()),
+        "rest_median": float(np.median(rest[rest > 0])) if np.any(rest > 0) else None,
+        "warm_median": float(np.median(warm[warm > 0])) if np.any(warm > 0) else None,
+        "rest_mean": float(np.mean(rest[ok])) if ok.any() else None,
+        "warm_mean": float(np.mean(warm[ok])) if ok.any() else None,
+        "ratio_median": float(np.median(ratio)) if ok.any() else None,
+        "ratio_of_means": float(np.mean(rest[ok]) / np.mean(warm[ok])) if ok.any() else None,
+        "inference_ms": infer_ms, "rest_steps": rest.tolist(), "warm_steps": warm.tolist(),
+    }
+
+
+def _train_main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m portlbm_amd.surrogate train")
+    ap.add_argument("--samples", type=int, default=20000)
+    ap.add_argument("--data-seed", type=int, default=20261202)
+    ap.add_argument("--solve-steps", type=int, default=8000)
+    ap.add_argument("--chunk", type=int, default=512)
+    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-min", type=float, default=2e-5)
+    ap.add_argument("--init-seed", type=int, default=7)
+    ap.add_argument("--eval", type=int, default=1000)
+    ap.add_argument("--eval-seed", type=int, default=99991)
+    ap.add_argument("--eval-max-steps", type=int, default=60000)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--report", required=True)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    size = 256
+    report: dict = {"args": vars(a)}
+    net = Surrogate(size, size, a.batch)
+    net.init_params(a.init_seed)
+    net.dataset_reserve(a.samples)
+    docs = porous_docs(a.samples, a.data_seed, size, storage="f32")
+    report["dataset"] = generate(net, docs, a.solve_steps, a.chunk, None, log=log)
+    j_scale = report["dataset"]["j_scale"]
+    n = net.dataset_size()
+    t0 = time.time()
+    hist = []
+    seg = 250
+    for s0 in range(0, a.train_steps, seg):
+        h = net.train(min(seg, a.train_steps - s0), a.train_steps, a.batch, a.lr, a.lr_min, seed=a.init_seed)
+        hist.append(h)
+        log(f"train: step {s0 + len(h)}/{a.train_steps}, loss {float(np.mean(h[-50:])):.5f}, "
+            f"{time.time() - t0:.1f} s")
+    hist = np.concatenate(hist)
+    report["train"] = {"samples": n, "steps": a.train_steps, "batch": a.batch, "seconds": time.time() - t0,
+                       "epochs": a.train_steps * a.batch / max(n, 1),
+                       "loss_first100": float(np.mean(hist[:100])), "loss_last100": float(np.mean(hist[-100:])),
+                       "loss_curve": hist[::max(1, len(hist) // 200)].tolist()}
+    model = Model(net.params(), j_scale, size, size,
+                  {"samples": n, "train_steps": a.train_steps, "data_seed": a.data_seed, "solve_steps": a.solve_steps})
+    model.save(a.out)
+    net.close()
+    if a.eval > 0:
+        eval_docs = porous_docs(a.eval, a.eval_seed, size, storage="f64")
+        report["eval"] = compare(eval_docs, model, max_steps=a.eval_max_steps, log=log)
+        e = report["eval"]
+        log(f"eval: rest median {e['rest_median']}, warm median {e['warm_median']}, ratio median "
+            f"{e['ratio_median']}, ratio of means {e['ratio_of_means']}, fallbacks {e['fallbacks']}")
+    Path(a.report).write_text(json.dumps(report, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "train":
+        sys.exit(_train_main(sys.argv[2:]))
+    print(__doc__)
+    sys.exit(2)

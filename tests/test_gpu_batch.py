@@ -130,3 +130,19 @@ def test_batch_equilibrium_warm_start_equals_single_lattice():
             b.init()
             for a, e in zip(got, expect):
                 assert np.array_equal(a, e), i
+
+
+def test_reset_domain_restarts_one_domain_from_rest():
+    docs = _small_batch(count=5, seed=17, storage="f32")
+    with pa.Batch(docs) as fresh:
+        fresh.run(60, "persistent")
+        expect = [fresh.macros(i) for i in range(len(docs))]
+    with pa.Batch(docs) as b:
+        b.run(101, "persistent")  # odd: the reset lands on the other lattice of the pair
+        before = [b.macros(i) for i in range(len(docs))]
+        b.reset_domain(2)
+        b.run(60, "persistent")
+        for a, e in zip(b.macros(2), expect[2]):
+            assert np.array_equal(a, e)
+        for i in (0, 1, 3, 4):  # untouched domains kept stepping
+            assert not np.array_equal(b.macros(i)[1], before[i][1])
