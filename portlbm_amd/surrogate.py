@@ -14,15 +14,16 @@ Encoding on an inner grid of ny x nx nodes (the network's H x W):
            1: x ramp, 2 (x + 0.5) / nx - 1
            2: (tau - 0.725) / 0.175
            3: (drho - 0.03) / 0.02, drho = rho_in - rho_out
-  targets  0: (rho - rho_out) / drho
+  targets  0: (rho - rho_out) / drho * rho_scale
            1, 2: j (tau - 1/2) / drho * j_scale
 The momentum normalisation is Stokes scaling: at low Reynolds number the
 steady momentum is proportional to the pressure drop over the viscosity,
 nu = (tau - 1/2) / 3, so the network mostly learns the geometry's share.
 j_scale (one number per model) brings the normalised momentum to unit RMS
-over the training set.
+over the training set; rho_scale > 1 weights the density in the loss.
 
-    python -m portlbm_amd.surrogate train --samples 20000 --out model.npz --report report.json
+    python -m portlbm_amd.surrogate train --out model.npz --report train.json
+    python -m portlbm_amd.surrogate evaluate --model model.npz --report eval.json
 """
 from __future__ import annotations
 
@@ -35,7 +36,7 @@ from pathlib import Path
 
 import numpy as np
 
-from . import Batch, LbmError, _check_hip, _ptr, build_phase, hip_library, settings
+from . import Batch, _check_hip, _ptr, build_phase, hip_library, settings
 
 INPUTS, OUTPUTS = 4, 3
 TAU_MID, TAU_HALF = 0.725, 0.175
@@ -207,27 +208,59 @@ def features(mask: np.ndarray, tau: float, rho_in: float, rho_out: float) -> np.
 
 
 def encode_targets(rho: np.ndarray, jx: np.ndarray, jy: np.ndarray, mask: np.ndarray, tau: float, rho_in: float,
-                   rho_out: float, j_scale: float) -> np.ndarray:
+                   rho_out: float, j_scale: float, rho_scale: float = 1.0) -> np.ndarray:
     ny, nx = mask.shape
     drho = rho_in - rho_out
     t = np.zeros((OUTPUTS, ny, nx), dtype=np.float32)
     s = (tau - 0.5) / drho * j_scale
-    t[0] = np.where(mask, (rho.reshape(ny, nx) - rho_out) / drho, 0.0)
+    t[0] = np.where(mask, (rho.reshape(ny, nx) - rho_out) / drho * rho_scale, 0.0)
     t[1] = np.where(mask, jx.reshape(ny, nx) * s, 0.0)
     t[2] = np.where(mask, jy.reshape(ny, nx) * s, 0.0)
     return t
 
 
-def decode(out: np.ndarray, mask: np.ndarray, tau: float, rho_in: float, rho_out: float,
-           j_scale: float) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-    """(density, x momentum, y momentum) on the inner grid, row-major; solid
-    nodes get the outlet density at rest (they are not written by the warm start)."""
+def trapped(mask: np.ndarray, axis: int) -> np.ndarray:
+    """Fluid nodes whose every population moving along `axis` is bounced back
+    (the three upstream nodes on both sides solid).  There the momentum along
+    `axis` only changes sign each step and never decays, and a start from
+    rest leaves it zero.  Beyond the inlet and outlet columns nothing is
+    solid, beyond the top and bottom rows everything is."""
+    solid = np.pad(~mask, ((1, 1), (0, 0)), constant_values=True)
+    solid = np.pad(solid, ((0, 0), (1, 1)), constant_values=False)
+    ny, nx = mask.shape
+    side = lambda dy, dx: solid[1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx]  # noqa: E731
+    t = np.ones_like(mask)
+    for k in (-1, 0, 1):
+        t &= (side(k, -1) & side(k, 1)) if axis == 1 else (side(-1, k) & side(1, k))
+    return t & mask
+
+
+def _masked_gaussian(a: np.ndarray, mask: np.ndarray, sigma: float) -> np.ndarray:
+    from scipy import ndimage
+    w = ndimage.gaussian_filter(mask.astype(np.float64), sigma)
+    v = ndimage.gaussian_filter(np.where(mask, a, 0.0), sigma)
+    return np.where(mask, v / np.maximum(w, 1e-12), a)
+
+
+def decode(out: np.ndarray, mask: np.ndarray, tau: float, rho_in: float, rho_out: float, j_scale: float,
+           rho_scale: float = 1.0, j_blend: float = 1.0,
+           rho_sigma: float = 0.0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(density, x momentum, y momentum) on the inner grid, row-major, for a
+    warm start.  Solid nodes get the outlet density at rest (the warm start
+    does not write them).  j_blend scales the predicted momentum (0: density
+    only), rho_sigma > 0 smooths the predicted density with a masked Gaussian
+    of that width, and the trapped momentum of one-node gaps is zeroed."""
     drho = rho_in - rho_out
-    s = drho / ((tau - 0.5) * j_scale)
+    s = drho / ((tau - 0.5) * j_scale) * j_blend
     o = out.astype(np.float64)
-    rho = np.where(mask, rho_out + drho * o[0], rho_out)
+    rho = rho_out + drho * o[0] / rho_scale
+    if rho_sigma > 0:
+        rho = _masked_gaussian(rho, mask, rho_sigma)
+    rho = np.where(mask, rho, rho_out)
     jx = np.where(mask, o[1] * s, 0.0)
     jy = np.where(mask, o[2] * s, 0.0)
+    jx[trapped(mask, 1)] = 0.0
+    jy[trapped(mask, 0)] = 0.0
     return rho.ravel(), jx.ravel(), jy.ravel()
 
 
@@ -235,27 +268,39 @@ def decode(out: np.ndarray, mask: np.ndarray, tau: float, rho_in: float, rho_out
 # model files (numpy .npz, no pickled objects)
 # ---------------------------------------------------------------------------
 class Model:
-    def __init__(self, params: np.ndarray, j_scale: float, height: int, width: int, meta: dict | None = None):
+    """Trained parameters, the target encoding (j_scale, rho_scale) and the
+    warm-start options chosen on validation domains (j_blend, rho_sigma)."""
+
+    def __init__(self, params: np.ndarray, height: int, width: int, j_scale: float, rho_scale: float = 1.0,
+                 j_blend: float = 1.0, rho_sigma: float = 0.0, meta: dict | None = None):
         self.params = _f32(params)
-        self.j_scale = float(j_scale)
         self.height, self.width = int(height), int(width)
+        self.j_scale, self.rho_scale = float(j_scale), float(rho_scale)
+        self.j_blend, self.rho_sigma = float(j_blend), float(rho_sigma)
         self.meta = meta or {}
 
     def save(self, path: str | Path) -> None:
-        np.savez(path, params=self.params, j_scale=np.float64(self.j_scale),
-                 shape=np.array([self.height, self.width], dtype=np.int64),
+        np.savez(path, params=self.params, shape=np.array([self.height, self.width], dtype=np.int64),
+                 encoding=np.array([self.j_scale, self.rho_scale, self.j_blend, self.rho_sigma]),
                  meta=np.array(json.dumps(self.meta)))
 
     @classmethod
     def load(cls, path: str | Path = DEFAULT_MODEL) -> "Model":
         with np.load(path, allow_pickle=False) as z:
             h, w = (int(v) for v in z["shape"])
-            return cls(z["params"], float(z["j_scale"]), h, w, json.loads(str(z["meta"])))
+            js, rs, jb, sg_ = (float(v) for v in z["encoding"])
+            return cls(z["params"], h, w, js, rs, jb, sg_, json.loads(str(z["meta"])))
 
     def network(self, max_batch: int = 16) -> Surrogate:
         net = Surrogate(self.height, self.width, max_batch)
         net.set_params(self.params)
         return net
+
+    def fields(self, out: np.ndarray, doc: dict, mask: np.ndarray, j_blend: float | None = None,
+               rho_sigma: float | None = None):
+        return decode(out, mask, *_physical(doc), self.j_scale, self.rho_scale,
+                      self.j_blend if j_blend is None else j_blend,
+                      self.rho_sigma if rho_sigma is None else rho_sigma)
 
 
 # ---------------------------------------------------------------------------
@@ -289,10 +334,11 @@ def solve_steady(docs: list[dict], steps: int, storage: str = "f32", final_check
 
 
 def generate(net: Surrogate, docs: list[dict], steps: int, chunk: int, j_scale: float | None,
-             max_residual: float = 1e-3, log=print) -> dict:
-    """Adds the steady states of `docs` to the network's device dataset.
-    Domains that diverge or whose residual stays above `max_residual` are
-    dropped.  j_scale None: taken from the first chunk (unit RMS)."""
+             rho_scale: float = 1.0, max_residual: float = 1e-3, log=print) -> dict:
+    """Adds the steady states of `docs` (FP32 batched solves of `steps`
+    steps) to the network's device dataset.  Domains that diverge or whose
+    residual stays above `max_residual` are dropped.  j_scale None: taken from
+    the first chunk (unit RMS of the normalised momentum)."""
     kept = dropped = 0
     t0 = time.time()
     for c0 in range(0, len(docs), chunk):
@@ -307,38 +353,44 @@ def generate(net: Surrogate, docs: list[dict], steps: int, chunk: int, j_scale: 
             mask = fluid_mask(doc)
             xs.append(features(mask, tau, rin, rout))
             raw.append((rho, jx, jy, mask, tau, rin, rout))
-        if j_scale is None:
+        if j_scale is None and raw:
             acc = [np.concatenate([(jx * (tau - 0.5) / (rin - rout))[mask.ravel()],
                                    (jy * (tau - 0.5) / (rin - rout))[mask.ravel()]])
                    for rho, jx, jy, mask, tau, rin, rout in raw]
             j_scale = float(1.0 / np.sqrt(np.mean(np.concatenate(acc) ** 2)))
-        ts = [encode_targets(*rw[:3], rw[3], rw[4], rw[5], rw[6], j_scale) for rw in raw]
+        ts = [encode_targets(*rw, j_scale, rho_scale) for rw in raw]
         if xs:
             net.dataset_add(np.stack(xs), np.stack(ts))
         kept += len(xs)
         log(f"dataset: {c0 + len(part)}/{len(docs)} solved, kept {kept}, dropped {dropped}, "
             f"{time.time() - t0:.1f} s")
-    return {"kept": kept, "dropped": dropped, "j_scale": j_scale, "seconds": time.time() - t0}
+    return {"kept": kept, "dropped": dropped, "j_scale": j_scale, "rho_scale": rho_scale,
+            "seconds": time.time() - t0}
 
 
 # ---------------------------------------------------------------------------
 # warm start and steps to steady state
 # ---------------------------------------------------------------------------
-def warm_start(batch: Batch, docs: list[dict], model: Model, net: Surrogate | None = None) -> float:
-    """Sets every domain of `batch` to the equilibrium of the predicted field;
-    returns the inference time (ms, HIP events)."""
+def predict(model: Model, docs: list[dict], net: Surrogate | None = None):
+    """Network outputs for `docs`, their fluid masks and the inference time (ms)."""
     own = net is None
     net = net or model.network()
     try:
         masks = [fluid_mask(d) for d in docs]
-        x = np.stack([features(m, *_physical(d)) for m, d in zip(masks, docs)])
-        out = net.predict(x)
-        ms = net.last_ms()
-        for i, (d, m) in enumerate(zip(docs, masks)):
-            batch.set_equilibrium(i, *decode(out[i], m, *_physical(d), model.j_scale))
+        out = net.predict(np.stack([features(m, *_physical(d)) for m, d in zip(masks, docs)]))
+        return out, masks, net.last_ms()
     finally:
         if own:
             net.close()
+
+
+def warm_start(batch: Batch, docs: list[dict], model: Model, net: Surrogate | None = None,
+               prediction=None, j_blend: float | None = None, rho_sigma: float | None = None) -> float:
+    """Sets every domain of `batch` to the equilibrium of the predicted field;
+    returns the inference time (ms, HIP events)."""
+    out, masks, ms = prediction if prediction is not None else predict(model, docs, net)
+    for i, (d, m) in enumerate(zip(docs, masks)):
+        batch.set_equilibrium(i, *model.fields(out[i], d, m, j_blend, rho_sigma))
     return ms
 
 
@@ -377,48 +429,80 @@ def steps_to_residual(batch: Batch, tol: float = 1e-6, check: int = 100, max_ste
     return {"steps": done, "fell_back": fell_back}
 
 
-def compare(docs: list[dict], model: Model, tol: float = 1e-6, max_steps: int = 60000, fallback_after: int = 20,
-            log=print) -> dict:
-    """Steps to `tol` from rest and from the warm start, same domains, FP64."""
-    with Batch(docs, storage="f64") as b:
-        t0 = time.time()
-        rest = steps_to_residual(b, tol, max_steps=max_steps)["steps"]
-        log(f"rest start: {time.time() - t0:.1f} s")
-    with Batch(docs, storage="f64") as b:
-        t0 = time.time()
-        infer_ms = warm_start(b, docs, model)
-        res = steps_to_residual(b, tol, max_steps=max_steps, fallback_after=fallback_after)
-        log(f"warm start: {time.time() - t0:.1f} s (inference {infer_ms:.1f} ms)")
-    warm = res["steps"]
+def _summary(rest: np.ndarray, warm: np.ndarray) -> dict:
     ok = (rest > 0) & (warm > 0)
     ratio = rest[ok] / warm[ok]
     return {
-        "domains": len(docs), "both_converged": int(ok.sum()), "rest_unconverged": int(np.sum(rest < 0)),
-        "warm_unconverged": int(np.sum(warm < 0)), "fallbacks": int(res["fell_back"].sum()),
+        "domains": int(rest.size), "both_converged": int(ok.sum()), "rest_unconverged": int(np.sum(rest < 0)),
+        "warm_unconverged": int(np.sum(warm < 0)),
         "rest_median": float(np.median(rest[rest > 0])) if np.any(rest > 0) else None,
         "warm_median": float(np.median(warm[warm > 0])) if np.any(warm > 0) else None,
         "rest_mean": float(np.mean(rest[ok])) if ok.any() else None,
         "warm_mean": float(np.mean(warm[ok])) if ok.any() else None,
         "ratio_median": float(np.median(ratio)) if ok.any() else None,
         "ratio_of_means": float(np.mean(rest[ok]) / np.mean(warm[ok])) if ok.any() else None,
-        "inference_ms": infer_ms, "rest_steps": rest.tolist(), "warm_steps": warm.tolist(),
     }
+
+
+def rest_steps(docs: list[dict], tol: float = 1e-6, max_steps: int = 60000) -> np.ndarray:
+    with Batch(docs, storage="f64") as b:
+        return steps_to_residual(b, tol, max_steps=max_steps)["steps"]
+
+
+def warm_steps(docs: list[dict], model: Model, prediction=None, tol: float = 1e-6, max_steps: int = 60000,
+               fallback_after: int | None = 20, **options) -> tuple[np.ndarray, dict]:
+    with Batch(docs, storage="f64") as b:
+        ms = warm_start(b, docs, model, prediction=prediction, **options)
+        r = steps_to_residual(b, tol, max_steps=max_steps, fallback_after=fallback_after)
+    return r["steps"], {"fallbacks": int(r["fell_back"].sum()), "inference_ms": ms}
+
+
+def compare(docs: list[dict], model: Model, tol: float = 1e-6, max_steps: int = 60000, fallback_after: int = 20,
+            log=print) -> dict:
+    """Steps to `tol` from rest and from the model's warm start, same domains, FP64."""
+    t0 = time.time()
+    rest = rest_steps(docs, tol, max_steps)
+    log(f"rest start: {time.time() - t0:.1f} s")
+    t0 = time.time()
+    warm, info = warm_steps(docs, model, None, tol, max_steps, fallback_after)
+    log(f"warm start: {time.time() - t0:.1f} s (inference {info['inference_ms']:.1f} ms)")
+    return {**_summary(rest, warm), **info, "rest_steps": rest.tolist(), "warm_steps": warm.tolist()}
+
+
+WARM_OPTIONS = [(jb, rs) for jb in (0.0, 0.5, 1.0) for rs in (0.0, 1.0, 2.0)]
+
+
+def select_options(docs: list[dict], model: Model, max_steps: int = 40000, log=print) -> dict:
+    """Warm-start options (j_blend, rho_sigma) with the highest mean-step gain
+    over rest on validation domains; sets them on `model`."""
+    rest = rest_steps(docs, max_steps=max_steps)
+    pred = predict(model, docs)
+    table = []
+    for jb, rs in WARM_OPTIONS:
+        warm, info = warm_steps(docs, model, pred, max_steps=max_steps, j_blend=jb, rho_sigma=rs)
+        row = {"j_blend": jb, "rho_sigma": rs, **_summary(rest, warm), "fallbacks": info["fallbacks"]}
+        table.append(row)
+        log(f"validation: j_blend {jb}, rho_sigma {rs}: ratio of means {row['ratio_of_means']}, "
+            f"ratio median {row['ratio_median']}, warm unconverged {row['warm_unconverged']}")
+    best = max(table, key=lambda r: (r["ratio_of_means"] or 0.0) - 0.01 * r["warm_unconverged"])
+    model.j_blend, model.rho_sigma = best["j_blend"], best["rho_sigma"]
+    return {"table": table, "chosen": {"j_blend": model.j_blend, "rho_sigma": model.rho_sigma}}
 
 
 def _train_main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m portlbm_amd.surrogate train")
     ap.add_argument("--samples", type=int, default=20000)
     ap.add_argument("--data-seed", type=int, default=20261202)
-    ap.add_argument("--solve-steps", type=int, default=8000)
+    ap.add_argument("--solve-steps", type=int, default=10000)
     ap.add_argument("--chunk", type=int, default=512)
-    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--rho-scale", type=float, default=4.0)
+    ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-min", type=float, default=2e-5)
     ap.add_argument("--init-seed", type=int, default=7)
-    ap.add_argument("--eval", type=int, default=1000)
-    ap.add_argument("--eval-seed", type=int, default=99991)
-    ap.add_argument("--eval-max-steps", type=int, default=60000)
+    ap.add_argument("--validation", type=int, default=128)
+    ap.add_argument("--validation-seed", type=int, default=5151)
     ap.add_argument("--out", required=True)
     ap.add_argument("--report", required=True)
     a = ap.parse_args(argv)
@@ -429,38 +513,55 @@ def _train_main(argv=None) -> int:
     net.init_params(a.init_seed)
     net.dataset_reserve(a.samples)
     docs = porous_docs(a.samples, a.data_seed, size, storage="f32")
-    report["dataset"] = generate(net, docs, a.solve_steps, a.chunk, None, log=log)
-    j_scale = report["dataset"]["j_scale"]
+    report["dataset"] = generate(net, docs, a.solve_steps, a.chunk, None, a.rho_scale, log=log)
     n = net.dataset_size()
     t0 = time.time()
     hist = []
-    seg = 250
+    seg = 500
     for s0 in range(0, a.train_steps, seg):
         h = net.train(min(seg, a.train_steps - s0), a.train_steps, a.batch, a.lr, a.lr_min, seed=a.init_seed)
         hist.append(h)
-        log(f"train: step {s0 + len(h)}/{a.train_steps}, loss {float(np.mean(h[-50:])):.5f}, "
+        log(f"train: step {s0 + len(h)}/{a.train_steps}, loss {float(np.mean(h[-100:])):.5f}, "
             f"{time.time() - t0:.1f} s")
     hist = np.concatenate(hist)
     report["train"] = {"samples": n, "steps": a.train_steps, "batch": a.batch, "seconds": time.time() - t0,
+                       "ms_per_step": (time.time() - t0) * 1e3 / a.train_steps,
                        "epochs": a.train_steps * a.batch / max(n, 1),
                        "loss_first100": float(np.mean(hist[:100])), "loss_last100": float(np.mean(hist[-100:])),
                        "loss_curve": hist[::max(1, len(hist) // 200)].tolist()}
-    model = Model(net.params(), j_scale, size, size,
-                  {"samples": n, "train_steps": a.train_steps, "data_seed": a.data_seed, "solve_steps": a.solve_steps})
-    model.save(a.out)
+    model = Model(net.params(), size, size, report["dataset"]["j_scale"], a.rho_scale,
+                  meta={"samples": n, "train_steps": a.train_steps, "data_seed": a.data_seed,
+                        "solve_steps": a.solve_steps, "validation_seed": a.validation_seed})
     net.close()
-    if a.eval > 0:
-        eval_docs = porous_docs(a.eval, a.eval_seed, size, storage="f64")
-        report["eval"] = compare(eval_docs, model, max_steps=a.eval_max_steps, log=log)
-        e = report["eval"]
-        log(f"eval: rest median {e['rest_median']}, warm median {e['warm_median']}, ratio median "
-            f"{e['ratio_median']}, ratio of means {e['ratio_of_means']}, fallbacks {e['fallbacks']}")
+    if a.validation > 0:
+        report["validation"] = select_options(porous_docs(a.validation, a.validation_seed, size), model, log=log)
+    model.save(a.out)
     Path(a.report).write_text(json.dumps(report, indent=1))
     return 0
 
 
+def _evaluate_main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m portlbm_amd.surrogate evaluate")
+    ap.add_argument("--model", default=str(DEFAULT_MODEL))
+    ap.add_argument("--domains", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=99991)
+    ap.add_argument("--max-steps", type=int, default=60000)
+    ap.add_argument("--report", required=True)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    model = Model.load(a.model)
+    docs = porous_docs(a.domains, a.seed, model.width)
+    res = compare(docs, model, max_steps=a.max_steps, log=log)
+    res["options"] = {"j_blend": model.j_blend, "rho_sigma": model.rho_sigma}
+    log(f"eval: rest median {res['rest_median']}, warm median {res['warm_median']}, ratio median "
+        f"{res['ratio_median']}, ratio of means {res['ratio_of_means']}, fallbacks {res['fallbacks']}")
+    Path(a.report).write_text(json.dumps({"args": vars(a), "eval": res}, indent=1))
+    return 0
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "train":
-        sys.exit(_train_main(sys.argv[2:]))
+    commands = {"train": _train_main, "evaluate": _evaluate_main}
+    if len(sys.argv) > 1 and sys.argv[1] in commands:
+        sys.exit(commands[sys.argv[1]](sys.argv[2:]))
     print(__doc__)
     sys.exit(2)
