@@ -30,6 +30,7 @@ sys.path.insert(0, str(ROOT))
 import numpy as np  # noqa: E402
 
 import portlbm_amd as pa  # noqa: E402
+from portlbm_amd import workloads  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
 INNER_H, INNER_V = 8192, 2048
@@ -125,9 +126,10 @@ def sweep(out_path: str, steps: int, warmup: int) -> None:
                                           "rows": rows}, indent=1))
 
 
-def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8)) -> None:
+def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: str = "cylinder") -> None:
     """Reduced-precision storage against FP64 on the 8192 x 2048 cylinder
-    channel (disc r = 64 at x = 1024, centred), inlet 1.02 / outlet 1.0.
+    channel (disc r = 64 at x = 1024, centred), inlet 1.02 / outlet 1.0, or
+    (scenario "porous") on the steady porous channel of the default workload.
     Errors are over fluid nodes, relative to the FP64 run of the same tau:
     L2 = ||a - a64||_2 / ||a64||_2 and Linf = max|a - a64| / max|a64| for the
     density and the momentum vector j; mass drift = sum rho(T) / sum rho(0) - 1
@@ -136,9 +138,10 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8)) -> None:
     for tau in taus:
         ref = None
         for storage in ("f64", "f32", "f16-shifted", "bf16-shifted"):
-            doc = pa.settings("nptl", "stream", inner_h=INNER_H, inner_v=INNER_V, scenario="cylinder",
-                              work_group_size=256, time_steps=steps, tau=tau, rho_in=1.02, rho_out=1.0,
-                              storage=storage, cylinderX=1024, cylinderRadius=64.0)
+            if scenario == "porous":
+                doc = workloads.porous_channel(storage=storage, steps=steps, tau=tau)
+            else:
+                doc = workloads.cylinder_channel(tau, storage=storage, steps=steps)
             lat = pa.Lattice.from_settings(doc)
             try:
                 fluid_mask = None
@@ -169,16 +172,16 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8)) -> None:
                 })
             rows.append(row)
             print(json.dumps(row), file=sys.stderr, flush=True)
-    Path(out_path).write_text(json.dumps({"workload": f"cylinder r=64 at x=1024 in {INNER_H}x{INNER_V}, "
-                                          "rho_in 1.02 / rho_out 1.0", "rows": rows}, indent=1))
+    workload = (f"porous-circles {INNER_H}x{INNER_V} (seed 20261202, radii U[4,16), porosity 0.7), "
+                "rho_in 1.05 / rho_out 1.0" if scenario == "porous" else
+                f"cylinder r=64 at x=1024 in {INNER_H}x{INNER_V}, rho_in 1.02 / rho_out 1.0")
+    Path(out_path).write_text(json.dumps({"workload": workload, "steps": steps, "rows": rows}, indent=1))
 
 
 def batched(out_path: str, steps: int, count: int = 512) -> dict:
     """512 uneven porous channels in FP32 (portlbm_amd.workloads.batch_channels):
     per-domain launches vs one persistent launch per step vs the whole run in
     one hipGraph with autotuned tiles."""
-    from portlbm_amd import workloads
-
     docs = workloads.batch_channels(count, steps=steps)
     rows = {}
     with pa.Batch(docs, storage="f32") as b:
@@ -217,6 +220,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep", metavar="OUT.json")
     ap.add_argument("--precision-study", metavar="OUT.json")
+    ap.add_argument("--study-scenario", default="cylinder", choices=("cylinder", "porous"))
+    ap.add_argument("--study-taus", default="0.52,0.6,0.8")
     ap.add_argument("--study-steps", type=int, default=20000)
     ap.add_argument("--batched", metavar="OUT.json")
     ap.add_argument("--batch-steps", type=int, default=2000)
@@ -230,7 +235,8 @@ def main() -> int:
     if args.sweep:
         sweep(args.sweep, min(args.steps, 500), min(args.warmup, 20))
     if args.precision_study:
-        precision_study(args.precision_study, args.study_steps)
+        taus = tuple(float(t) for t in args.study_taus.split(","))
+        precision_study(args.precision_study, args.study_steps, taus, args.study_scenario)
     if args.batched:
         batched(args.batched, args.batch_steps)
 

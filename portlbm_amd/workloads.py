@@ -17,10 +17,11 @@ from . import settings
 
 
 def porous_channel(algorithm: str = "nptl", layout: str = "stream", storage: str = "f64", steps: int = 5000,
-                   scenario: str = "porous-circles", inner_h: int = 8192, inner_v: int = 2048) -> dict:
+                   scenario: str = "porous-circles", inner_h: int = 8192, inner_v: int = 2048,
+                   tau: float = 0.6) -> dict:
     wg = 1024 if algorithm in ("npol", "nsol") else 256
     return settings(algorithm, layout, inner_h=inner_h, inner_v=inner_v, scenario=scenario, work_group_size=wg,
-                    time_steps=steps, tau=0.6, rho_in=1.05, rho_out=1.0, storage=storage, seed=20261202,
+                    time_steps=steps, tau=tau, rho_in=1.05, rho_out=1.0, storage=storage, seed=20261202,
                     porosity=0.7, radiusMin=4.0, radiusMax=16.0, obstacleMargin=32)
 
 
