@@ -137,11 +137,14 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: 
     rows = []
     for tau in taus:
         ref = None
-        for storage in ("f64", "f32", "f16-shifted", "bf16-shifted"):
+        for variant in ("f64", "f32", "f16-shifted", "f16-shifted/stochastic", "bf16-shifted",
+                        "bf16-shifted/stochastic"):
+            storage, _, rounding = variant.partition("/")
             if scenario == "porous":
                 doc = workloads.porous_channel(storage=storage, steps=steps, tau=tau)
             else:
                 doc = workloads.cylinder_channel(tau, storage=storage, steps=steps)
+            doc["mi355x"]["rounding"] = rounding or "nearest"
             lat = pa.Lattice.from_settings(doc)
             try:
                 fluid_mask = None
@@ -152,7 +155,8 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: 
             finally:
                 lat.close()
             fluid = rho > -1
-            row = {"tau": tau, "storage": storage, "steps": steps, "fluid_nodes": int(fluid.sum()),
+            row = {"tau": tau, "storage": storage, "rounding": rounding or "nearest", "steps": steps,
+                   "fluid_nodes": int(fluid.sum()),
                    "mflups": round(int(fluid.sum()) * steps / (ms * 1e-3) / 1e6, 1),
                    "mass_drift": red["mass"] / m0 - 1.0, "drag_x": red["fx"], "drag_y": red["fy"],
                    "finite": bool(np.isfinite(rho[fluid]).all() and np.isfinite(jx[fluid]).all())}

@@ -21,6 +21,8 @@ struct Extensions
 {
     // population storage on the GPU: "f64", "f32", "f16-shifted", "bf16-shifted"
     std::string storage = "f64";
+    // rounding of the FP16/BF16 population stores: "nearest" or "stochastic"
+    std::string rounding = "nearest";
     // outputs that are NaN or above FLT_MAX are written as 0 (the reference's
     // WITH_NAN_PROTECTION build option, ON by default)
     bool nan_protection = true;

@@ -72,6 +72,13 @@ enum lbm_storage {
     LBM_STORE_BF16_SHIFTED = 3
 };
 
+/* Rounding of the FP16/BF16 population stores in the step kernels (FP64 and
+ * FP32 ignore it).  Stochastic: rounded up with probability equal to the
+ * remainder, the random bits a hash of (node place, direction, iteration) --
+ * deterministic, and unbiased where round-to-nearest repeats the same error
+ * every step near a steady state.  No counterpart in the reference. */
+enum lbm_rounding { LBM_ROUND_NEAREST = 0, LBM_ROUND_STOCHASTIC = 1 };
+
 /* Node-code bits returned by lbm_get_node_codes (one uint16 per expanded node).
  * Bit d (0..8) set: the population pulled along direction d comes from a
  * solid node, so half-way bounce-back supplies the node's own post-collision
@@ -101,7 +108,7 @@ typedef struct lbm_desc {
     uint32_t subdomain_count_horizontal;
     uint32_t subdomain_count_vertical;
     uint32_t work_group_size;
-    uint32_t reserved0;
+    uint32_t rounding;      /* enum lbm_rounding: stores of the FP16/BF16 forms in the step kernels */
     double relaxation_time;
     double inlet_density;
     double inlet_velocity_x;

@@ -69,7 +69,33 @@ struct Storage<LBM_STORE_F32>
     static __device__ __forceinline__ double decode(T v, int) { return double(v); }
 };
 
-// FP16 deviation storage: HBM holds f_i - w_i as IEEE binary16 (RNE).
+// 32-bit integer hash (xor-shift-multiply finaliser); random bits for
+// stochastic rounding.
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// binary32 -> binary16, stochastic: the 13 bits binary16 drops are added to
+// the 13 random bits r (< 2^13) and cut, so |v| rounds up with probability
+// equal to the dropped fraction.  Exact for the binary16 normal range; below
+// it the conversion's own rounding applies to the remainder.  Inf/NaN pass
+// through.
+__device__ __forceinline__ _Float16 f32_to_f16_sr(float v, uint32_t r)
+{
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7f800000u) != 0x7f800000u)
+        u = (u + r) & ~0x1fffu;
+    return _Float16(__uint_as_float(u));
+}
+
+// FP16 deviation storage: HBM holds f_i - w_i as IEEE binary16 (RNE, or
+// stochastic through store_sr).
 template <>
 struct Storage<LBM_STORE_F16_SHIFTED>
 {
@@ -87,6 +113,14 @@ struct Storage<LBM_STORE_F16_SHIFTED>
     {
         __builtin_nontemporal_store(_Float16(v - weight<float>(d)), p + i);
     }
+    static __device__ __forceinline__ void store_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    {
+        p[i] = f32_to_f16_sr(v - weight<float>(d), r);
+    }
+    static __device__ __forceinline__ void store_nt_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    {
+        __builtin_nontemporal_store(f32_to_f16_sr(v - weight<float>(d), r), p + i);
+    }
     static __device__ __forceinline__ T encode(double v, int d) { return _Float16(float(v - weight<double>(d))); }
     static __device__ __forceinline__ double decode(T v, int d) { return double(float(v)) + weight<double>(d); }
 };
@@ -101,6 +135,16 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f)
     return uint16_t(u >> 16);
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+// binary32 -> bfloat16, stochastic (16 dropped bits against the 16 random
+// bits r < 2^16).
+__device__ __forceinline__ uint16_t f32_to_bf16_sr(float f, uint32_t r)
+{
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u)
+        return f32_to_bf16(f);
+    return uint16_t((u + r) >> 16);
+}
 
 // BF16 deviation storage: HBM holds f_i - w_i as bfloat16 (RNE).
 template <>
@@ -119,6 +163,14 @@ struct Storage<LBM_STORE_BF16_SHIFTED>
     static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int d)
     {
         __builtin_nontemporal_store(f32_to_bf16(v - weight<float>(d)), p + i);
+    }
+    static __device__ __forceinline__ void store_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    {
+        p[i] = f32_to_bf16_sr(v - weight<float>(d), r);
+    }
+    static __device__ __forceinline__ void store_nt_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    {
+        __builtin_nontemporal_store(f32_to_bf16_sr(v - weight<float>(d), r), p + i);
     }
     static __device__ __forceinline__ T encode(double v, int d) { return f32_to_bf16(float(v - weight<double>(d))); }
     static __device__ __forceinline__ double decode(T v, int d) { return double(bf16_to_f32(v)) + weight<double>(d); }
@@ -163,6 +215,7 @@ struct StepParams
     uint32_t P, X0, S;    // physical placement
     uint32_t sub_h, sub_v;
     uint32_t write_macros;
+    uint32_t sr;          // stochastic-rounding seed of this step (0: round to nearest)
     uint32_t nan_protection;
 };
 
@@ -241,10 +294,33 @@ __device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C
 // displace the pull's halo lines in L2.  In the collision and bundle layouts
 // one instruction writes every 9th (3rd) element; those stores go through L2,
 // where the nine (three) instructions of a node row combine into full lines.
+// sr: 0 rounds to nearest; otherwise the per-step seed of stochastic rounding
+// (FP16/BF16 only).  The random bits of place p: h = mix32(sr ^ p), then per
+// direction one step of a 32-bit LCG (PCG's multiplier and increment), of
+// which the top 13 (FP16) or 16 (BF16) bits are used.
 template <int L, int S>
 __device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t p, uint32_t Sd,
-                                           const typename Storage<S>::C f[9])
+                                           const typename Storage<S>::C f[9], uint32_t sr = 0)
 {
+    if constexpr (S == LBM_STORE_F16_SHIFTED || S == LBM_STORE_BF16_SHIFTED)
+    {
+        if (sr)
+        {
+            constexpr int drop = S == LBM_STORE_F16_SHIFTED ? 19 : 16;
+            uint32_t h = mix32(sr ^ p);
+#pragma unroll
+            for (int d = 0; d < 9; ++d)
+            {
+                h = h * 747796405u + 2891336453u;
+                const uint32_t r = h >> drop;
+                if constexpr (L == LBM_LAYOUT_STREAM)
+                    Storage<S>::store_nt_sr(dst, at<L>(p, uint32_t(d), Sd), f[d], d, r);
+                else
+                    Storage<S>::store_sr(dst, at<L>(p, uint32_t(d), Sd), f[d], d, r);
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int d = 0; d < 9; ++d)
     {

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
         finish_node<L, S>(p, code, x, y, f);
     }
     if (fluid || fill)
-        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f);
+        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr);
 }
 
 // lptl: two lattices, one linear range over every node of the lattice
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(1024) void k_step_lptl(StepParams p)
         finish_node<L, S>(p, code, x, y, f);
     }
     if (fluid || fill)
-        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f);
+        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr);
 }
 
 // npol: one lattice, one work-group per subdomain tile; every pull of the
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(1024) void k_step_npol(StepParams p)
     if (fluid)
         finish_node<L, S>(p, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
     if (fluid || fill)
-        store_node<L, S>(lat, pn, p.S, f);
+        store_node<L, S>(lat, pn, p.S, f, p.sr);
 }
 
 // nsol: one lattice, swap scheme (nsol_kernels.hpp:171-334).  Work-groups of
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
         finish_node<L, S>(p, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
     }
     if (fluid || fill)
-        store_node<L, S>(lat, pn, p.S, f);
+        store_node<L, S>(lat, pn, p.S, f, p.sr);
 }
 
 // Buffer rows: directions 0..2 from the row above, 6..8 from the row below
@@ -403,6 +403,14 @@ __global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, siz
 // ---------------------------------------------------------------------------
 // dispatch over layout x storage
 // ---------------------------------------------------------------------------
+inline uint32_t step_seed(uint64_t iteration)
+{
+    uint64_t z = iteration + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return uint32_t(z ^ (z >> 31));
+}
+
 template <int L, int S>
 struct Impl
 {
@@ -436,6 +444,8 @@ struct Impl
         p.sub_v = c->sub_v;
         p.write_macros = macros ? 1u : 0u;
         p.nan_protection = c->desc.nan_protection ? 1u : 0u;
+        // per-step seed, never 0 (0 selects round-to-nearest)
+        p.sr = c->desc.rounding == LBM_ROUND_STOCHASTIC ? (step_seed(c->iteration) | 1u) : 0u;
 
         switch (c->desc.algorithm)
         {
@@ -621,6 +631,8 @@ static int validate_desc(const lbm_desc *d)
         return fail(LBM_ERR_INVALID_ARGUMENT, "unknown data layout");
     if (d->storage < LBM_STORE_F64 || d->storage > LBM_STORE_BF16_SHIFTED)
         return fail(LBM_ERR_INVALID_ARGUMENT, "unknown storage type");
+    if (d->rounding != LBM_ROUND_NEAREST && d->rounding != LBM_ROUND_STOCHASTIC)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "unknown rounding mode");
     if (d->horizontal_nodes < 4 || d->vertical_nodes < 4)
         return fail(LBM_ERR_INVALID_ARGUMENT, "horizontal_nodes and vertical_nodes must be >= 4");
     if (!(d->relaxation_time > 0.0))

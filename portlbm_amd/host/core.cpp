@@ -27,7 +27,7 @@ namespace core
 bool Extensions::is_default() const
 {
     const Extensions d;
-    return storage == d.storage && nan_protection == d.nan_protection && seed == d.seed && porosity == d.porosity &&
+    return storage == d.storage && rounding == d.rounding && nan_protection == d.nan_protection && seed == d.seed && porosity == d.porosity &&
            radius_min == d.radius_min && radius_max == d.radius_max && obstacle_margin == d.obstacle_margin &&
            max_attempts == d.max_attempts &&
            cylinder_x == d.cylinder_x && cylinder_radius == d.cylinder_radius;
@@ -104,6 +104,9 @@ void Properties::validate() const
     if (!in_list(extensions.storage, constants::storage_types))
         throw PropertyArgumentException("Unknown storage: \"" + extensions.storage +
                                         "\". Valid values: f64, f32, f16-shifted, bf16-shifted");
+    if (extensions.rounding != "nearest" && extensions.rounding != "stochastic")
+        throw PropertyArgumentException("Unknown rounding: \"" + extensions.rounding +
+                                        "\". Valid values: nearest, stochastic");
     if (!(extensions.porosity > 0.0 && extensions.porosity <= 1.0))
         throw PropertyArgumentException("porosity must be in (0, 1]");
     if (!(extensions.radius_min > 0.0 && extensions.radius_max >= extensions.radius_min))
@@ -135,7 +138,7 @@ std::string Properties::to_string() const
        << "\tOutlet velocity: (" << outlet_velocity_x << ", " << outlet_velocity_y << ")\n"
        << "\tOutlet density: " << outlet_density << "\n"
        << "\tRelaxation time: " << relaxation_time << "\n"
-       << "Storage: " << extensions.storage << "\n\n";
+       << "Storage: " << extensions.storage << " (rounding: " << extensions.rounding << ")\n\n";
     return os.str();
 }
 

@@ -132,6 +132,7 @@ void HipAlgorithmHandler::initialize_algorithm()
     desc.layout = layout;
     desc.storage = storage;
     desc.nan_protection = properties_->extensions.nan_protection ? 1 : 0;
+    desc.rounding = properties_->extensions.rounding == "stochastic" ? LBM_ROUND_STOCHASTIC : LBM_ROUND_NEAREST;
     desc.horizontal_nodes = properties_->horizontal_nodes;
     desc.vertical_nodes = properties_->vertical_nodes;
     desc.expanded_horizontal_nodes = domain_->horizontal_nodes;

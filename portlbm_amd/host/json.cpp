@@ -529,6 +529,7 @@ JsonValue properties_to_json_value(const core::Properties &p)
         const core::Extensions &e = p.extensions;
         JsonValue &x = doc["mi355x"];
         x["storage"] = JsonValue::string(e.storage);
+        x["rounding"] = JsonValue::string(e.rounding);
         x["nanProtection"] = JsonValue::boolean(e.nan_protection);
         x["seed"] = JsonValue::unsigned_number(e.seed);
         x["porosity"] = JsonValue::number(e.porosity);
@@ -578,6 +579,8 @@ core::Properties json_value_to_properties(const JsonValue &data, int offset)
         core::Extensions &e = p.extensions;
         if (const JsonValue *f = x->find("storage"))
             e.storage = f->as_string("mi355x.storage");
+        if (const JsonValue *f = x->find("rounding"))
+            e.rounding = f->as_string("mi355x.rounding");
         if (const JsonValue *f = x->find("nanProtection"))
             e.nan_protection = f->as_bool("mi355x.nanProtection");
         if (const JsonValue *f = x->find("seed"))

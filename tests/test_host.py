@@ -60,6 +60,17 @@ def test_roundtrip_keeps_the_reference_schema():
     assert back["physical"]["relaxationTime"] == 0.75
     ext = pa.roundtrip(pa.settings(storage="f32", seed=5))
     assert ext["mi355x"]["storage"] == "f32" and ext["mi355x"]["seed"] == 5
+    assert ext["mi355x"]["rounding"] == "nearest"
+    sr = pa.roundtrip(pa.settings(storage="f16-shifted", rounding="stochastic"))
+    assert sr["mi355x"]["rounding"] == "stochastic"
+    assert "rounding: stochastic" in pa.properties_string(pa.settings(storage="f16-shifted", rounding="stochastic"))
+    assert pa.desc_from_settings(pa.settings(rounding="stochastic")).rounding == 1
+
+
+def test_validate_rejects_unknown_rounding():
+    with pytest.raises(pa.LbmError) as e:
+        pa.validate(pa.settings(rounding="upward"))
+    assert e.value.code == 1
 
 
 @pytest.mark.parametrize("field,value", [("algorithm", "xptl"), ("dataLayout", "soa"), ("timeSteps", 0),
