@@ -68,6 +68,16 @@ class HipAlgorithmHandler : public AlgorithmHandler
     const std::vector<std::int8_t> &get_phase() const { return phase_; }
     std::vector<double> get_populations() const;
     lbm_ctx *context() const { return ctx_; }
+    // fluid nodes of the node-type table (the unit of MFLUPS)
+    std::size_t fluid_node_count() const { return fluid_nodes_; }
+    // GPU time of the steps run since the last initialize() (HIP events), seconds
+    double device_run_seconds() const { return device_ms_ * 1e-3; }
+    // name of the GPU the handler runs on
+    static std::string device_name();
+
+    // Seconds the last initialize() took (the reference's BENCHMARK_MODE
+    // member, sycl_algorithm_handler.hpp:252-271,281).
+    double initialization_time = 0.0;
 
   private:
     void initialize_algorithm();
@@ -83,6 +93,8 @@ class HipAlgorithmHandler : public AlgorithmHandler
     std::vector<std::int8_t> phase_;
     mutable std::vector<real_type> densities_, x_velocities_, y_velocities_, absolute_velocities_;
     lbm_ctx *ctx_ = nullptr;
+    std::size_t fluid_nodes_ = 0;
+    double device_ms_ = 0.0;
     std::future<void> future_;
     bool active_ = false;
 };

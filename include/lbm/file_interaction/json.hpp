@@ -54,6 +54,7 @@ class JsonValue
     // Array access.
     void push_back(JsonValue v);
     const std::vector<JsonValue> &items() const { return array_; }
+    JsonValue &item(std::size_t i) { return array_.at(i); }
 
     std::string dump(int indent = 4) const;
     static JsonValue parse(const std::string &text);  // throws UnknownPropertyException on syntax errors

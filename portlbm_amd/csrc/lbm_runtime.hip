@@ -623,6 +623,21 @@ int lbm_device_max_work_group_size(uint32_t *out)
     return LBM_OK;
 }
 
+int lbm_device_name(char *buf, size_t len)
+{
+    if (!buf || len == 0)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_device_name: empty buffer");
+    int dev = 0;
+    LBM_HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop{};
+    LBM_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    const std::string name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    const size_t n = std::min(len - 1, name.size());
+    std::memcpy(buf, name.data(), n);
+    buf[n] = '\0';
+    return LBM_OK;
+}
+
 static int validate_desc(const lbm_desc *d)
 {
     if (d->algorithm < LBM_ALG_NPTL || d->algorithm > LBM_ALG_NSOL)

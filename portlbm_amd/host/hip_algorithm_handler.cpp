@@ -1,5 +1,6 @@
 // HipAlgorithmHandler: the reference's handler lifecycle on the GPU library.
 #include "lbm/execution/hip_algorithm_handler.hpp"
+#include "lbm/core/timer.hpp"
 
 #include <algorithm>
 #include <string>
@@ -99,6 +100,7 @@ void HipAlgorithmHandler::release()
 
 void HipAlgorithmHandler::initialize_algorithm()
 {
+    core::Timer timer;
     core::Properties props =
         explicit_props_.has_value() ? *explicit_props_ : file_interaction::json_to_properties(settings_path_);
     const int alg = algorithm_id(props.algorithm);
@@ -153,6 +155,10 @@ void HipAlgorithmHandler::initialize_algorithm()
     check(lbm_create(&desc, &ctx_), "lbm_create");
     check(lbm_set_phase(ctx_, phase_.data(), phase_.size()), "lbm_set_phase");
     check(lbm_init_populations(ctx_), "lbm_init_populations");
+    check(lbm_sync(ctx_), "lbm_sync");
+    fluid_nodes_ = static_cast<std::size_t>(std::count(phase_.begin(), phase_.end(), std::int8_t{0}));
+    device_ms_ = 0.0;
+    initialization_time = timer.elapsed();
     // a ready future, so that block_until_finished() before start() returns
     // at once, as in the reference (sycl_algorithm.hpp:50-53)
     future_ = std::async(std::launch::deferred, [] {});
@@ -185,6 +191,9 @@ void HipAlgorithmHandler::execute()
         const unsigned int batch = std::min({remaining, to_frame, max_batch});
         control_->reset_timer();
         check(lbm_run(ctx_, batch, fui), "lbm_run");
+        double ms = 0.0;
+        check(lbm_last_run_ms(ctx_, &ms), "lbm_last_run_ms");
+        device_ms_ += ms;
         control_->finalize_iterations(batch, control_->elapsed_ms());
         const unsigned int last = it + batch - 1;
         if (fui && last % fui == fui - 1)
@@ -230,6 +239,13 @@ void HipAlgorithmHandler::block_until_finished()
     }
     check(lbm_sync(ctx_), "lbm_sync");
     active_ = false;
+}
+
+std::string HipAlgorithmHandler::device_name()
+{
+    char buf[256];
+    check(lbm_device_name(buf, sizeof buf), "lbm_device_name");
+    return buf;
 }
 
 std::vector<double> HipAlgorithmHandler::get_populations() const

@@ -1,9 +1,15 @@
-// Command-line driver: portlbm [settings.json] (reference: src/main_gpu.cpp:88-99,186-202).
-// Runs the configured simulation to the end and prints its throughput.
+// Command-line driver (reference: src/main_gpu.cpp:88-99,151-202).
+//   portlbm [settings.json]               runs the configured simulation to the end
+//   portlbm --benchmark [settings.json]   the three-phase benchmark (the reference's
+//                                         BENCHMARK_MODE build); results go to
+//                                         <project>/benchmarks/phase{0,1,2}/, <project>
+//                                         being the settings file's grandparent directory
 #include <filesystem>
 #include <iostream>
 #include <string>
 
+#include "lbm/benchmark/benchmark.hpp"
+#include "lbm/execution/hip_algorithm_handler.hpp"
 #include "lbm/portlbm.hpp"
 
 namespace
@@ -30,10 +36,27 @@ std::string resolve_settings(int argc, const char *const *argv)
 }
 }  // namespace
 
+int run_benchmark(const std::string &settings_path)
+{
+    const std::filesystem::path root = std::filesystem::absolute(settings_path).parent_path().parent_path();
+    for (const char *phase : {"phase0", "phase1", "phase2"})
+        std::filesystem::create_directories(root / "benchmarks" / phase);
+    std::cout << "Starting benchmark.\n\n";
+    auto handler = std::make_shared<lbm::execution::HipAlgorithmHandler>(settings_path);
+    lbm::benchmark::Benchmark benchmark(handler);
+    benchmark.phase_0();
+    benchmark.phase_1();
+    benchmark.phase_2();
+    std::cout << "\nDone with all benchmarks. Exiting.\n\n";
+    return 0;
+}
+
 int main(int argc, char *argv[])
 {
     try
     {
+        if (argc > 1 && std::string(argv[1]) == "--benchmark")
+            return run_benchmark(resolve_settings(argc - 1, argv + 1));
         const std::string settings_path = resolve_settings(argc, argv);
         auto handler = lbm::create_handler(settings_path);
         lbm::core::Timer timer;

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "lbm/benchmark/benchmark.hpp"
 #include "lbm/core/access.hpp"
 #include "lbm/core/domain.hpp"
 #include "lbm/core/properties.hpp"
@@ -478,6 +479,52 @@ void json_errors()
     CHECK(JsonValue::parse(v.dump()).at("f").as_double("f") == -1.5e-3);
 }
 TEST_CASE("JSON reader: syntax errors, escapes, exact integers", json_errors);
+
+// benchmark.cpp:30-62 for a device maximum of 1024 (gfx950): per quarter, the
+// powers of two plus the first odd and the first even other size.
+void benchmark_work_group_sizes()
+{
+    using lbm::benchmark::get_work_group_sizes;
+    const std::vector<std::size_t> expect = {1024, 1023, 1022, 768, 767, 512, 511, 510, 256, 255, 254};
+    CHECK(get_work_group_sizes(1024) == expect);
+    const std::vector<std::size_t> small = {16, 15, 14, 12, 11, 8, 7, 6, 4, 3, 2};
+    CHECK(get_work_group_sizes(16) == small);
+}
+TEST_CASE("Benchmark: work-group sizes per device maximum", benchmark_work_group_sizes);
+
+void benchmark_progress_and_time_format()
+{
+    lbm::benchmark::Progress p(4);
+    p.advance(2.0);
+    CHECK(p.current_test_rows == 1 && p.progress == 25.0 && p.current_average_time == 2.0);
+    CHECK(p.remaining_time_estimation == 6.0);
+    p.advance(4.0);
+    CHECK(p.current_average_time == 3.0 && p.remaining_time_estimation == 6.0 && p.progress == 50.0);
+    CHECK(lbm::benchmark::seconds_to_time_format(90061) == "1d, 1h, 1m, 1s");
+    CHECK(lbm::benchmark::seconds_to_time_format(59) == "0d, 0h, 0m, 59s");
+}
+TEST_CASE("Benchmark: progress and remaining-time format", benchmark_progress_and_time_format);
+
+void benchmark_settings_file()
+{
+    const std::filesystem::path path = std::filesystem::path(g_settings_path).parent_path() / "benchmark.json";
+    const auto s = lbm::benchmark::BenchmarkSettings::from_file(path.string());
+    CHECK(s.test_runs == 10 && s.test_linear_two_lattice);
+    CHECK((s.data_layouts == std::vector<std::string>{"collision", "stream", "bundle"}));
+    CHECK((s.algorithms == std::vector<std::string>{"nptl", "npol", "nsol"}));
+    CHECK((s.phase_2_scenarios == std::vector<std::string>{"plate", "wing", "porous"}));
+    bool threw = false;
+    try
+    {
+        lbm::benchmark::BenchmarkSettings::from_file("/nonexistent/benchmark.json");
+    }
+    catch (const lbm::exceptions::Exception &)
+    {
+        threw = true;
+    }
+    CHECK(threw);
+}
+TEST_CASE("Benchmark: settings/benchmark.json", benchmark_settings_file);
 
 }  // namespace
 
