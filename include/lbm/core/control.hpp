@@ -40,6 +40,13 @@ class Control
             last_frametime_.store(elapsed_ms / count);
     }
 
+    // Continue from `it` completed iterations (a restored snapshot).
+    void set_current_iteration(unsigned int it)
+    {
+        current_iteration_.store(it);
+        progress_.store(static_cast<double>(it) / static_cast<double>(max_iterations_));
+    }
+
     void reset_timer() { timer_.restart(); }
     double elapsed_ms() const { return timer_.elapsed() * 1e3; }
     unsigned int get_current_iteration() const { return current_iteration_.load(); }

@@ -75,6 +75,18 @@ class HipAlgorithmHandler : public AlgorithmHandler
     // name of the GPU the handler runs on
     static std::string device_name();
 
+    // Snapshot file of the current state (settings + lattice state of the C
+    // ABI's lbm_get_state): "PLBMSNAP", u32 version 1, u32 header length, a
+    // JSON header {"settings", "iteration", "stateBytes"}, the state.  Loading
+    // requires equal settings apart from timeSteps and frameUpdateInterval
+    // (PropertyArgumentException otherwise) and continues bit for bit.  Not
+    // while the worker runs.
+    void save_snapshot(const std::string &path) const;
+    void load_snapshot(const std::string &path);
+    // The four result fields as CSV: header "x,y,density,x_velocity,y_velocity,
+    // absolute_velocity", one row per inner node (x, y from 1, row-major).
+    void export_csv(const std::string &path) const;
+
     // Seconds the last initialize() took (the reference's BENCHMARK_MODE
     // member, sycl_algorithm_handler.hpp:252-271,281).
     double initialization_time = 0.0;

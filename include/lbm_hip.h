@@ -178,6 +178,16 @@ int lbm_get_macros(lbm_ctx *ctx, double *density, double *x_velocity, double *y_
 int lbm_get_populations(lbm_ctx *ctx, double *values, size_t count);
 int lbm_set_populations(lbm_ctx *ctx, const double *values, size_t count);
 
+/* The lattice state as the device holds it, for snapshots and restarts: a
+ * 16-byte head (iteration u64, source-lattice index u32, lattice count u32),
+ * every lattice in its storage type and physical placement, the outlet's uy
+ * history (2 V doubles) and the four result arrays (4 (H-2)(V-2) doubles).
+ * Loading it into a context created from the same description (and node
+ * types) continues the run bit for bit.  No counterpart in the reference. */
+int lbm_state_size(const lbm_ctx *ctx, size_t *bytes);
+int lbm_get_state(lbm_ctx *ctx, void *buf, size_t bytes);
+int lbm_set_state(lbm_ctx *ctx, const void *buf, size_t bytes);
+
 /* Overwrite every fluid node with the equilibrium of a given (density,
  * momentum) field given on the inner grid ((H-2)*(V-2) values each, same
  * order as lbm_get_macros).  Used by warm starts.  No reference counterpart. */

@@ -60,6 +60,17 @@ int portlbm_get_last_frametime(portlbm_handler *h, double *ms);
 int portlbm_get_phase(portlbm_handler *h, int8_t *out, size_t count);
 int portlbm_get_populations(portlbm_handler *h, double *out, size_t count);
 
+/* Snapshot of the handler's state to a file / restore from one (no
+ * counterpart in the reference; HipAlgorithmHandler::save_snapshot /
+ * load_snapshot): loading needs the same settings apart from timeSteps and
+ * frameUpdateInterval and continues the run bit for bit.  Not while running. */
+int portlbm_save_snapshot(portlbm_handler *h, const char *path);
+int portlbm_load_snapshot(portlbm_handler *h, const char *path);
+/* The four result fields as CSV (x, y, density, x_velocity, y_velocity,
+ * absolute_velocity per inner node) -- the GUI's "results.csv" option
+ * (include/lbm/gui/lbm_gui.hpp:106-107), which the reference leaves unwritten. */
+int portlbm_export_csv(portlbm_handler *h, const char *path);
+
 /* Properties::validate() of the settings in json_text. */
 int portlbm_validate_json(const char *json_text);
 /* Domain extents of the settings, same order as portlbm_get_extents. */

@@ -152,6 +152,9 @@ def host_library() -> ctypes.CDLL:
             "portlbm_build_phase": ([ctypes.c_char_p, _P, _SZ], ctypes.c_int),
             "portlbm_roundtrip_json": ([ctypes.c_char_p, _P, _SZ, ctypes.POINTER(_SZ)], ctypes.c_int),
             "portlbm_properties_string": ([ctypes.c_char_p, _P, _SZ, ctypes.POINTER(_SZ)], ctypes.c_int),
+            "portlbm_save_snapshot": ([_P, ctypes.c_char_p], ctypes.c_int),
+            "portlbm_load_snapshot": ([_P, ctypes.c_char_p], ctypes.c_int),
+            "portlbm_export_csv": ([_P, ctypes.c_char_p], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -490,6 +493,15 @@ class Handler:
         out = np.empty(self.extents["N"], dtype=np.int8)
         _check_host(self.lib.portlbm_get_phase(self._h, _ptr(out), out.size), "get_phase")
         return out
+
+    def save_snapshot(self, path: str | Path) -> None:
+        _check_host(self.lib.portlbm_save_snapshot(self._h, str(path).encode()), "portlbm_save_snapshot")
+
+    def load_snapshot(self, path: str | Path) -> None:
+        _check_host(self.lib.portlbm_load_snapshot(self._h, str(path).encode()), "portlbm_load_snapshot")
+
+    def export_csv(self, path: str | Path) -> None:
+        _check_host(self.lib.portlbm_export_csv(self._h, str(path).encode()), "portlbm_export_csv")
 
     def populations(self) -> np.ndarray:
         out = np.empty(9 * self.extents["N"])

@@ -917,6 +917,75 @@ int lbm_get_macros(lbm_ctx *c, double *rho, double *jx, double *jy, double *jabs
     return LBM_OK;
 }
 
+static size_t state_bytes(const lbm_ctx *c)
+{
+    const size_t lattices = c->two_lattice ? 2 : 1;
+    return 16 + lattices * c->lattice_elems * c->elem + 2ull * c->V * sizeof(double) + 4ull * c->D * sizeof(double);
+}
+
+int lbm_state_size(const lbm_ctx *c, size_t *bytes)
+{
+    if (!c || !bytes)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_state_size: null argument");
+    *bytes = state_bytes(c);
+    return LBM_OK;
+}
+
+int lbm_get_state(lbm_ctx *c, void *buf, size_t bytes)
+{
+    if (!c || !buf)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_state: null argument");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_get_state: call lbm_init_populations first");
+    if (bytes != state_bytes(c))
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_state: size must be lbm_state_size()");
+    char *out = static_cast<char *>(buf);
+    const uint32_t lattices = c->two_lattice ? 2u : 1u;
+    const uint32_t cur = uint32_t(c->cur);
+    std::memcpy(out, &c->iteration, 8);
+    std::memcpy(out + 8, &cur, 4);
+    std::memcpy(out + 12, &lattices, 4);
+    size_t off = 16;
+    const size_t lat_bytes = c->lattice_elems * c->elem;
+    for (uint32_t l = 0; l < lattices; ++l, off += lat_bytes)
+        LBM_HIP_TRY(hipMemcpyAsync(out + off, c->lat[l], lat_bytes, hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP_TRY(hipMemcpyAsync(out + off, c->uycol, 2ull * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    off += 2ull * c->V * sizeof(double);
+    LBM_HIP_TRY(hipMemcpyAsync(out + off, c->res, 4ull * c->D * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_set_state(lbm_ctx *c, const void *buf, size_t bytes)
+{
+    if (!c || !buf)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_state: null argument");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_set_state: call lbm_init_populations first");
+    if (bytes != state_bytes(c))
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_state: size does not match this context");
+    const char *in = static_cast<const char *>(buf);
+    uint64_t iteration = 0;
+    uint32_t cur = 0, lattices = 0;
+    std::memcpy(&iteration, in, 8);
+    std::memcpy(&cur, in + 8, 4);
+    std::memcpy(&lattices, in + 12, 4);
+    if (lattices != (c->two_lattice ? 2u : 1u) || cur >= lattices)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_state: state of another algorithm");
+    size_t off = 16;
+    const size_t lat_bytes = c->lattice_elems * c->elem;
+    for (uint32_t l = 0; l < lattices; ++l, off += lat_bytes)
+        LBM_HIP_TRY(hipMemcpyAsync(c->lat[l], in + off, lat_bytes, hipMemcpyHostToDevice, c->stream));
+    LBM_HIP_TRY(hipMemcpyAsync(c->uycol, in + off, 2ull * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    off += 2ull * c->V * sizeof(double);
+    LBM_HIP_TRY(hipMemcpyAsync(c->res, in + off, 4ull * c->D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->iteration = iteration;
+    c->cur = int(cur);
+    c->snap_valid = false;
+    return LBM_OK;
+}
+
 int lbm_get_populations(lbm_ctx *c, double *values, size_t count)
 {
     if (!c || !values)

@@ -246,6 +246,33 @@ int portlbm_get_populations(portlbm_handler *h, double *out, size_t count)
     });
 }
 
+int portlbm_save_snapshot(portlbm_handler *h, const char *path)
+{
+    return guarded([&] {
+        need(h);
+        need(path);
+        h->impl->save_snapshot(path);
+    });
+}
+
+int portlbm_load_snapshot(portlbm_handler *h, const char *path)
+{
+    return guarded([&] {
+        need(h);
+        need(path);
+        h->impl->load_snapshot(path);
+    });
+}
+
+int portlbm_export_csv(portlbm_handler *h, const char *path)
+{
+    return guarded([&] {
+        need(h);
+        need(path);
+        h->impl->export_csv(path);
+    });
+}
+
 int portlbm_validate_json(const char *json_text)
 {
     return guarded([&] { parse(json_text).validate(); });

@@ -2,6 +2,9 @@
 #include "lbm/execution/hip_algorithm_handler.hpp"
 #include "lbm/core/timer.hpp"
 
+#include <cstring>
+#include <fstream>
+#include <cstdio>
 #include <algorithm>
 #include <string>
 
@@ -239,6 +242,105 @@ void HipAlgorithmHandler::block_until_finished()
     }
     check(lbm_sync(ctx_), "lbm_sync");
     active_ = false;
+}
+
+namespace
+{
+constexpr char kSnapshotMagic[8] = {'P', 'L', 'B', 'M', 'S', 'N', 'A', 'P'};
+
+// settings that must agree between a snapshot and the handler loading it
+file_interaction::JsonValue comparable_settings(const core::Properties &p)
+{
+    file_interaction::JsonValue v = file_interaction::properties_to_json_value(p);
+    v["algorithmic"]["timeSteps"] = file_interaction::JsonValue::unsigned_number(0);
+    v["algorithmic"]["frameUpdateInterval"] = file_interaction::JsonValue::unsigned_number(0);
+    return v;
+}
+}  // namespace
+
+void HipAlgorithmHandler::save_snapshot(const std::string &path) const
+{
+    if (active_)
+        throw exceptions::Exception("save_snapshot: pause or finish the run first");
+    check(lbm_sync(ctx_), "lbm_sync");
+    size_t bytes = 0;
+    check(lbm_state_size(ctx_, &bytes), "lbm_state_size");
+    std::vector<char> state(bytes);
+    check(lbm_get_state(ctx_, state.data(), bytes), "lbm_get_state");
+    file_interaction::JsonValue head = file_interaction::JsonValue::object();
+    head["settings"] = file_interaction::properties_to_json_value(*properties_);
+    head["iteration"] = file_interaction::JsonValue::unsigned_number(control_->get_current_iteration());
+    head["stateBytes"] = file_interaction::JsonValue::unsigned_number(bytes);
+    const std::string text = head.dump(0);
+    std::ofstream out(path, std::ios::binary);
+    if (!out)
+        throw exceptions::Exception("cannot write snapshot \"" + path + "\"");
+    const std::uint32_t version = 1, len = static_cast<std::uint32_t>(text.size());
+    out.write(kSnapshotMagic, 8);
+    out.write(reinterpret_cast<const char *>(&version), 4);
+    out.write(reinterpret_cast<const char *>(&len), 4);
+    out.write(text.data(), static_cast<std::streamsize>(text.size()));
+    out.write(state.data(), static_cast<std::streamsize>(state.size()));
+    if (!out)
+        throw exceptions::Exception("writing snapshot \"" + path + "\" failed");
+}
+
+void HipAlgorithmHandler::load_snapshot(const std::string &path)
+{
+    if (active_)
+        throw exceptions::Exception("load_snapshot: pause or finish the run first");
+    std::ifstream in(path, std::ios::binary);
+    if (!in)
+        throw exceptions::Exception("cannot read snapshot \"" + path + "\"");
+    char magic[8];
+    std::uint32_t version = 0, len = 0;
+    in.read(magic, 8);
+    in.read(reinterpret_cast<char *>(&version), 4);
+    in.read(reinterpret_cast<char *>(&len), 4);
+    if (!in || std::memcmp(magic, kSnapshotMagic, 8) != 0 || version != 1 || len > (1u << 24))
+        throw exceptions::Exception("\"" + path + "\" is not a snapshot of this library");
+    std::string text(len, '\0');
+    in.read(text.data(), len);
+    const file_interaction::JsonValue head = file_interaction::JsonValue::parse(text);
+    // node counts are stored with their ghost layers (properties_to_json_value)
+    const core::Properties saved = file_interaction::json_value_to_properties(head.at("settings"), -2);
+    if (comparable_settings(saved).dump(0) != comparable_settings(*properties_).dump(0))
+        throw exceptions::json::PropertyArgumentException(
+            "the snapshot was taken with other settings (only timeSteps and frameUpdateInterval may differ)");
+    const std::uint64_t iteration = head.at("iteration").as_unsigned("iteration");
+    if (iteration > control_->get_max_iterations())
+        throw exceptions::json::PropertyArgumentException("the snapshot is past this run's timeSteps");
+    size_t bytes = 0;
+    check(lbm_state_size(ctx_, &bytes), "lbm_state_size");
+    if (head.at("stateBytes").as_unsigned("stateBytes") != bytes)
+        throw exceptions::json::PropertyArgumentException("the snapshot's state size does not match");
+    std::vector<char> state(bytes);
+    in.read(state.data(), static_cast<std::streamsize>(bytes));
+    if (!in)
+        throw exceptions::Exception("snapshot \"" + path + "\" is truncated");
+    check(lbm_set_state(ctx_, state.data(), bytes), "lbm_set_state");
+    control_->set_current_iteration(static_cast<unsigned int>(iteration));
+    copy_macroscopic_observables_to_cpu();
+}
+
+void HipAlgorithmHandler::export_csv(const std::string &path) const
+{
+    std::ofstream out(path);
+    if (!out)
+        throw exceptions::Exception("cannot write \"" + path + "\"");
+    const unsigned int w = properties_->horizontal_nodes - 2, h = properties_->vertical_nodes - 2;
+    out << "x,y,density,x_velocity,y_velocity,absolute_velocity\n";
+    char line[192];
+    for (unsigned int y = 0; y < h; ++y)
+        for (unsigned int x = 0; x < w; ++x)
+        {
+            const std::size_t i = x + static_cast<std::size_t>(y) * w;
+            std::snprintf(line, sizeof line, "%u,%u,%.17g,%.17g,%.17g,%.17g\n", x + 1, y + 1, densities_[i],
+                          x_velocities_[i], y_velocities_[i], absolute_velocities_[i]);
+            out << line;
+        }
+    if (!out)
+        throw exceptions::Exception("writing \"" + path + "\" failed");
 }
 
 std::string HipAlgorithmHandler::device_name()

@@ -1,0 +1,80 @@
+"""Snapshots (save/load of the whole lattice state through the handler and
+the C ABI) continue a run bit for bit; CSV export writes the result fields."""
+import numpy as np
+import pytest
+
+import portlbm_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+
+def _doc(alg, layout, storage, steps, rounding="nearest", **kw):
+    wg = 64 if alg in ("npol", "nsol") else 128
+    return pa.settings(alg, layout, inner_h=96, inner_v=40, scenario="circle", work_group_size=wg, time_steps=steps,
+                       frame_update_interval=50, tau=0.7, rho_in=1.05, storage=storage, rounding=rounding, **kw)
+
+
+@pytest.mark.parametrize("alg,layout,storage,rounding", [
+    ("nptl", "stream", "f64", "nearest"), ("lptl", "bundle", "f32", "nearest"),
+    ("npol", "collision", "f64", "nearest"), ("nsol", "stream", "f16-shifted", "stochastic"),
+    ("nptl", "collision", "bf16-shifted", "nearest")])
+def test_restart_continues_bit_for_bit(tmp_path, alg, layout, storage, rounding):
+    with pa.Handler(_doc(alg, layout, storage, 300, rounding)) as ref:
+        ref.start()
+        ref.block_until_finished()
+        expect = [ref.field(n) for n in ("density", "x_velocity", "y_velocity", "absolute_velocity")]
+        expect_pop = ref.populations()
+    snap = tmp_path / "s.plbm"
+    with pa.Handler(_doc(alg, layout, storage, 123, rounding)) as first:
+        first.start()
+        first.block_until_finished()
+        first.save_snapshot(snap)
+        at_save = first.field("density")
+    with pa.Handler(_doc(alg, layout, storage, 300, rounding)) as second:
+        second.load_snapshot(snap)
+        assert second.get_current_iteration() == 123
+        np.testing.assert_array_equal(second.field("density"), at_save)  # results restored too
+        second.start()
+        second.block_until_finished()
+        assert second.get_current_iteration() == 300
+        for n, e in zip(("density", "x_velocity", "y_velocity", "absolute_velocity"), expect):
+            np.testing.assert_array_equal(second.field(n), e, err_msg=n)
+        np.testing.assert_array_equal(second.populations(), expect_pop)
+
+
+def test_snapshot_refuses_other_settings_and_other_files(tmp_path):
+    snap = tmp_path / "s.plbm"
+    with pa.Handler(_doc("nptl", "stream", "f64", 20)) as h:
+        h.start()
+        h.block_until_finished()
+        h.save_snapshot(snap)
+    with pa.Handler(pa.settings("nptl", "stream", inner_h=96, inner_v=40, scenario="circle", work_group_size=128,
+                                time_steps=50, tau=0.8, rho_in=1.05)) as other:
+        with pytest.raises(pa.LbmError) as e:
+            other.load_snapshot(snap)
+        assert e.value.code == 1
+    with pa.Handler(_doc("nptl", "stream", "f64", 10)) as short:
+        with pytest.raises(pa.LbmError):  # snapshot at 20 > timeSteps 10
+            short.load_snapshot(snap)
+        bad = tmp_path / "bad.plbm"
+        bad.write_bytes(b"not a snapshot at all")
+        with pytest.raises(pa.LbmError):
+            short.load_snapshot(bad)
+        with pytest.raises(pa.LbmError):
+            short.load_snapshot(tmp_path / "missing.plbm")
+
+
+def test_csv_export_matches_fields(tmp_path):
+    with pa.Handler(_doc("lptl", "stream", "f64", 200)) as h:
+        h.start()
+        h.block_until_finished()
+        path = tmp_path / "results.csv"
+        h.export_csv(path)
+        lines = path.read_text().splitlines()
+        assert lines[0] == "x,y,density,x_velocity,y_velocity,absolute_velocity"
+        data = np.loadtxt(path, delimiter=",", skiprows=1)
+        assert data.shape == (96 * 40, 6)
+        np.testing.assert_array_equal(data[:, 0], np.tile(np.arange(1, 97), 40))
+        np.testing.assert_array_equal(data[:, 1], np.repeat(np.arange(1, 41), 96))
+        for k, n in enumerate(("density", "x_velocity", "y_velocity", "absolute_velocity")):
+            np.testing.assert_array_equal(data[:, 2 + k], h.field(n))  # %.17g round-trips
