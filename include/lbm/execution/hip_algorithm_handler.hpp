@@ -87,6 +87,13 @@ class HipAlgorithmHandler : public AlgorithmHandler
     // absolute_velocity", one row per inner node (x, y from 1, row-major).
     void export_csv(const std::string &path) const;
 
+    // Debug mode (settings debugMode): the fields of every finished iteration,
+    // appended in order (the reference's all_densities / all_x_velocities /
+    // all_y_velocities, nptl.hpp:170-175), each (H-2)(V-2) values per step.
+    const std::vector<real_type> &debug_densities() const { return debug_densities_; }
+    const std::vector<real_type> &debug_x_velocities() const { return debug_x_velocities_; }
+    const std::vector<real_type> &debug_y_velocities() const { return debug_y_velocities_; }
+
     // Seconds the last initialize() took (the reference's BENCHMARK_MODE
     // member, sycl_algorithm_handler.hpp:252-271,281).
     double initialization_time = 0.0;
@@ -94,7 +101,10 @@ class HipAlgorithmHandler : public AlgorithmHandler
   private:
     void initialize_algorithm();
     void release();
-    void execute();  // worker body
+    void execute();        // worker body
+    void execute_debug();  // worker body in debug mode: split steps
+    static void pass_callback(void *user, int pass, std::uint64_t iteration);
+    void print_pass(int pass, std::uint64_t iteration);
     void copy_macroscopic_observables_to_cpu();
 
     std::string settings_path_;
@@ -104,6 +114,7 @@ class HipAlgorithmHandler : public AlgorithmHandler
     std::unique_ptr<core::Control> control_;
     std::vector<std::int8_t> phase_;
     mutable std::vector<real_type> densities_, x_velocities_, y_velocities_, absolute_velocities_;
+    std::vector<real_type> debug_densities_, debug_x_velocities_, debug_y_velocities_;
     lbm_ctx *ctx_ = nullptr;
     std::size_t fluid_nodes_ = 0;
     double device_ms_ = 0.0;

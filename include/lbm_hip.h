@@ -159,6 +159,24 @@ int lbm_get_node_codes(lbm_ctx *ctx, uint16_t *codes, size_t node_count);
  * every step in its own column.  frame_interval 0 means "last step only". */
 int lbm_run(lbm_ctx *ctx, uint32_t steps, uint32_t frame_interval);
 
+/* Passes of a split (debug) step, the reference's debug kernels
+ * (nptl.hpp:165-514, nptl_kernels.hpp:48-295): pull streaming with half-way
+ * bounce-back, the Zou-He outlet, the moments (results and outlet history
+ * written every step), the collision.  Between passes the populations live in
+ * an FP64 pass buffer.  A split step performs the fused step's arithmetic in
+ * the same order and equals it bit for bit, for every algorithm, layout and
+ * storage. */
+enum lbm_pass { LBM_PASS_STREAM = 0, LBM_PASS_OUTLET = 1, LBM_PASS_MACROSCOPIC = 2, LBM_PASS_COLLIDE = 3 };
+/* Called after each pass with the iteration count (after the collide pass it
+ * counts the finished step); the callback may read the pass buffer
+ * (lbm_get_pass_populations), the results (lbm_get_macros) and, after the
+ * collide pass, the new lattice (lbm_get_populations). */
+typedef void (*lbm_pass_callback)(void *user, int pass, uint64_t iteration);
+int lbm_run_split(lbm_ctx *ctx, uint32_t steps, lbm_pass_callback cb, void *user);
+/* The pass buffer, direction-major over logical nodes: values[d * N + n] for
+ * N the expanded node count (fluid nodes only are defined). */
+int lbm_get_pass_populations(lbm_ctx *ctx, double *values, size_t count);
+
 /* Steps completed since lbm_init_populations. */
 int lbm_get_iteration(lbm_ctx *ctx, uint64_t *iteration);
 

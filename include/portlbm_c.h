@@ -71,6 +71,13 @@ int portlbm_load_snapshot(portlbm_handler *h, const char *path);
  * (include/lbm/gui/lbm_gui.hpp:106-107), which the reference leaves unwritten. */
 int portlbm_export_csv(portlbm_handler *h, const char *path);
 
+/* Debug mode (debugMode in the settings): the fields of every finished
+ * iteration, (H-2)(V-2) values per step in order (the reference's
+ * all_densities / all_x_velocities / all_y_velocities, nptl.hpp:170-175).
+ * field: PORTLBM_DENSITY, PORTLBM_X_VELOCITY or PORTLBM_Y_VELOCITY; *needed
+ * receives the value count; out may be NULL to query it. */
+int portlbm_debug_history(portlbm_handler *h, int field, double *out, size_t count, size_t *needed);
+
 /* Properties::validate() of the settings in json_text. */
 int portlbm_validate_json(const char *json_text);
 /* Domain extents of the settings, same order as portlbm_get_extents. */

@@ -77,6 +77,9 @@ _hip: ctypes.CDLL | None = None
 _host: ctypes.CDLL | None = None
 
 _P = ctypes.c_void_p
+# lbm_pass_callback of include/lbm_hip.h
+PASS_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64)
+PASSES = ("stream", "outlet", "macroscopic", "collide")
 _U32 = ctypes.c_uint32
 _SZ = ctypes.c_size_t
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -104,6 +107,8 @@ def hip_library() -> ctypes.CDLL:
             "lbm_init_populations": ([_P], ctypes.c_int),
             "lbm_get_node_codes": ([_P, _P, _SZ], ctypes.c_int),
             "lbm_run": ([_P, _U32, _U32], ctypes.c_int),
+            "lbm_run_split": ([_P, _U32, PASS_CALLBACK, _P], ctypes.c_int),
+            "lbm_get_pass_populations": ([_P, _P, _SZ], ctypes.c_int),
             "lbm_get_iteration": ([_P, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "lbm_last_run_ms": ([_P, _DP], ctypes.c_int),
             "lbm_get_macros": ([_P, _P, _P, _P, _P, _SZ], ctypes.c_int),
@@ -155,6 +160,7 @@ def host_library() -> ctypes.CDLL:
             "portlbm_save_snapshot": ([_P, ctypes.c_char_p], ctypes.c_int),
             "portlbm_load_snapshot": ([_P, ctypes.c_char_p], ctypes.c_int),
             "portlbm_export_csv": ([_P, ctypes.c_char_p], ctypes.c_int),
+            "portlbm_debug_history": ([_P, ctypes.c_int, _P, _SZ, ctypes.POINTER(_SZ)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -373,6 +379,19 @@ class Lattice:
         values = np.ascontiguousarray(values, dtype=np.float64)
         _check_hip(self.lib.lbm_set_populations(self._ctx, _ptr(values), values.size), "lbm_set_populations")
 
+    def run_split(self, steps: int, callback=None) -> float:
+        """Split (debug) steps; callback(pass, iteration) after every pass."""
+        cb = PASS_CALLBACK((lambda _u, p, it: callback(p, it)) if callback else (lambda _u, p, it: None))
+        _check_hip(self.lib.lbm_run_split(self._ctx, steps, cb, None), "lbm_run_split")
+        ms = ctypes.c_double(0)
+        _check_hip(self.lib.lbm_last_run_ms(self._ctx, ctypes.byref(ms)), "lbm_last_run_ms")
+        return ms.value
+
+    def pass_populations(self) -> np.ndarray:
+        out = np.empty(9 * self.N)
+        _check_hip(self.lib.lbm_get_pass_populations(self._ctx, _ptr(out), out.size), "lbm_get_pass_populations")
+        return out.reshape(9, self.N)
+
     def set_equilibrium_field(self, rho: np.ndarray, jx: np.ndarray, jy: np.ndarray) -> None:
         arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (rho, jx, jy)]
         _check_hip(self.lib.lbm_set_equilibrium_field(self._ctx, *map(_ptr, arrs), self.D),
@@ -499,6 +518,16 @@ class Handler:
 
     def load_snapshot(self, path: str | Path) -> None:
         _check_host(self.lib.portlbm_load_snapshot(self._h, str(path).encode()), "portlbm_load_snapshot")
+
+    def debug_history(self, name: str) -> np.ndarray:
+        """Debug mode: the field of every finished iteration, shape (steps, (H-2)(V-2))."""
+        n = _SZ(0)
+        _check_host(self.lib.portlbm_debug_history(self._h, self.FIELDS[name], None, 0, ctypes.byref(n)),
+                    "debug_history")
+        out = np.empty(n.value)
+        _check_host(self.lib.portlbm_debug_history(self._h, self.FIELDS[name], _ptr(out), out.size,
+                                                   ctypes.byref(n)), "debug_history")
+        return out
 
     def export_csv(self, path: str | Path) -> None:
         _check_host(self.lib.portlbm_export_csv(self._h, str(path).encode()), "portlbm_export_csv")

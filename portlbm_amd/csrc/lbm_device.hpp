@@ -266,7 +266,7 @@ __device__ __forceinline__ void zou_he_outlet(C f[9], C rho_out, C uy_prev)
 // f' = -(1/tau)(f - feq) + f.  Velocity is the momentum j (never divided by
 // rho).  Term order as in nptl_kernels.hpp:382-405.
 template <class C>
-__device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C &usq)
+__device__ __forceinline__ void moments(const C f[9], C &rho, C &jx, C &jy, C &usq)
 {
     rho = C(0);
     jx = C(0);
@@ -279,6 +279,12 @@ __device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C
         jy += f[d] * C(cy_of(d));
     }
     usq = jx * jx + jy * jy;
+}
+
+template <class C>
+__device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C &usq)
+{
+    moments<C>(f, rho, jx, jy, usq);
 #pragma unroll
     for (int d = 0; d < 9; ++d)
     {
@@ -352,15 +358,12 @@ __device__ __forceinline__ bool segment_has(bool pred)
 // Finish one fluid node whose pulled populations are in f: outlet, collision,
 // outlet-uy history and (on frame steps) macroscopic outputs at the inner
 // index of unexpanded (x, y).  The caller stores f.
-template <int L, int S>
-__device__ __forceinline__ void finish_node(const StepParams &p, uint32_t code, uint32_t x, uint32_t y,
-                                            typename Storage<S>::C f[9])
+// The outlet's uy input (UY_SOURCE nodes, every step) and, on macro steps,
+// the four results of inner node (x, y) (unexpanded coordinates).
+template <class C>
+__device__ __forceinline__ void write_moments(const StepParams &p, uint32_t code, uint32_t x, uint32_t y, C rho, C jx,
+                                              C jy, C usq)
 {
-    using C = typename Storage<S>::C;
-    if (code & LBM_CODE_OUTLET)
-        zou_he_outlet<C>(f, C(p.rho_out), C(p.uy_in[y]));
-    C rho, jx, jy, usq;
-    collide<C>(f, C(p.omega), rho, jx, jy, usq);
     if (code & LBM_CODE_UY_SOURCE)
         p.uy_out[y] = double(clamp_output(jy, p.nan_protection));
     if (p.write_macros)
@@ -372,6 +375,18 @@ __device__ __forceinline__ void finish_node(const StepParams &p, uint32_t code, 
         p.jy[r] = double(clamp_output(jy, p.nan_protection));
         p.jabs[r] = double(clamp_output(jabs, p.nan_protection));
     }
+}
+
+template <int L, int S>
+__device__ __forceinline__ void finish_node(const StepParams &p, uint32_t code, uint32_t x, uint32_t y,
+                                            typename Storage<S>::C f[9])
+{
+    using C = typename Storage<S>::C;
+    if (code & LBM_CODE_OUTLET)
+        zou_he_outlet<C>(f, C(p.rho_out), C(p.uy_in[y]));
+    C rho, jx, jy, usq;
+    collide<C>(f, C(p.omega), rho, jx, jy, usq);
+    write_moments<C>(p, code, x, y, rho, jx, jy, usq);
 }
 
 }  // namespace lbm_dev

@@ -66,6 +66,7 @@ struct lbm_ctx
     double *res = nullptr;    // 4 * D: rho, jx, jy, |j|
     double *uycol = nullptr;  // 2 * V outlet uy history (ping-pong)
     double *snap = nullptr;   // 2 * N residual snapshot (lazy)
+    double *pass = nullptr;   // 9 * P * Vx FP64 pass buffer of split steps (lazy)
     double *partials = nullptr;
     double *red_out = nullptr;
     uint32_t D;               // (H-2)*(V-2)
@@ -401,6 +402,110 @@ __global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, siz
 }
 
 // ---------------------------------------------------------------------------
+// split step (the reference's debug kernels, nptl_kernels.hpp:48-295): one
+// launch per pass over every physical place, the populations between passes
+// kept in an FP64 pass buffer pass[d * PV + place] (PV = P * Vx).  Each pass
+// performs the fused kernel's arithmetic in the same order, so a split step
+// equals a fused step bit for bit in every storage type.
+// ---------------------------------------------------------------------------
+// unexpanded coordinates of a physical place (buffer rows/columns mapped out)
+__device__ __forceinline__ void split_xy(const StepParams &p, uint32_t pn, uint32_t buffered, uint32_t &x,
+                                         uint32_t &y)
+{
+    y = pn / p.P;
+    x = pn - y * p.P - p.X0;
+    if (buffered)
+    {
+        x -= x / (p.sub_h + 1u);
+        y -= y / (p.sub_v + 1u);
+    }
+}
+
+template <int L, int S>
+__global__ void k_split_stream(StepParams p, double *pass, uint32_t PV)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    const uint32_t pn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pn >= PV)
+        return;
+    const uint32_t code = p.code[pn];
+    if (!(code & LBM_CODE_FLUID))
+        return;
+    C f[9];
+    pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f);
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+        pass[size_t(d) * PV + pn] = double(f[d]);
+}
+
+template <int S>
+__global__ void k_split_outlet(StepParams p, double *pass, uint32_t PV, uint32_t buffered)
+{
+    using C = typename Storage<S>::C;
+    const uint32_t pn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pn >= PV)
+        return;
+    const uint32_t code = p.code[pn];
+    if (!(code & LBM_CODE_FLUID) || !(code & LBM_CODE_OUTLET))
+        return;
+    uint32_t x, y;
+    split_xy(p, pn, buffered, x, y);
+    C f[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+        f[d] = C(pass[size_t(d) * PV + pn]);
+    zou_he_outlet<C>(f, C(p.rho_out), C(p.uy_in[y]));
+    pass[pn] = double(f[0]);
+    pass[3ull * PV + pn] = double(f[3]);
+    pass[6ull * PV + pn] = double(f[6]);
+}
+
+template <int S>
+__global__ void k_split_moments(StepParams p, double *pass, uint32_t PV, uint32_t buffered)
+{
+    using C = typename Storage<S>::C;
+    const uint32_t pn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pn >= PV)
+        return;
+    const uint32_t code = p.code[pn];
+    if (!(code & LBM_CODE_FLUID))
+        return;
+    uint32_t x, y;
+    split_xy(p, pn, buffered, x, y);
+    C f[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d)
+        f[d] = C(pass[size_t(d) * PV + pn]);
+    C rho, jx, jy, usq;
+    moments<C>(f, rho, jx, jy, usq);
+    write_moments<C>(p, code, x, y, rho, jx, jy, usq);
+}
+
+template <int L, int S>
+__global__ void k_split_collide(StepParams p, const double *pass, uint32_t PV)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    const uint32_t pn = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t code = pn < PV ? p.code[pn] : 0u;
+    const bool fluid = code & LBM_CODE_FLUID;
+    const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
+    const bool fill = (code & LBM_CODE_SOLID) && segment;
+    C f[9] = {};
+    if (fluid)
+    {
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            f[d] = C(pass[size_t(d) * PV + pn]);
+        C rho, jx, jy, usq;
+        collide<C>(f, C(p.omega), rho, jx, jy, usq);
+    }
+    if (fluid || fill)
+        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr);
+}
+
+// ---------------------------------------------------------------------------
 // dispatch over layout x storage
 // ---------------------------------------------------------------------------
 inline uint32_t step_seed(uint64_t iteration)
@@ -417,7 +522,7 @@ struct Impl
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
 
-    static int step(lbm_ctx *c, bool macros)
+    static StepParams params(lbm_ctx *c, bool macros)
     {
         StepParams p{};
         p.src = c->lat[c->cur];
@@ -446,7 +551,12 @@ struct Impl
         p.nan_protection = c->desc.nan_protection ? 1u : 0u;
         // per-step seed, never 0 (0 selects round-to-nearest)
         p.sr = c->desc.rounding == LBM_ROUND_STOCHASTIC ? (step_seed(c->iteration) | 1u) : 0u;
+        return p;
+    }
 
+    static int step(lbm_ctx *c, bool macros)
+    {
+        const StepParams p = params(c, macros);
         switch (c->desc.algorithm)
         {
         case LBM_ALG_NPTL:
@@ -480,6 +590,39 @@ struct Impl
         if (c->two_lattice)
             c->cur ^= 1;
         c->iteration++;
+        return LBM_OK;
+    }
+
+    // One pass of a split step (pass = enum lbm_pass); the collide pass ends
+    // the step (lattice swap, iteration count).
+    static int split(lbm_ctx *c, int pass, double *buf)
+    {
+        const StepParams p = params(c, true);
+        const uint32_t PV = c->P * c->Vx;
+        const dim3 grid((PV + 255u) / 256u), block(256);
+        const uint32_t buffered = c->buffered ? 1u : 0u;
+        switch (pass)
+        {
+        case LBM_PASS_STREAM:
+            if (c->buffered)
+                refresh(c);
+            hipLaunchKernelGGL((k_split_stream<L, S>), grid, block, 0, c->stream, p, buf, PV);
+            break;
+        case LBM_PASS_OUTLET:
+            hipLaunchKernelGGL((k_split_outlet<S>), grid, block, 0, c->stream, p, buf, PV, buffered);
+            break;
+        case LBM_PASS_MACROSCOPIC:
+            hipLaunchKernelGGL((k_split_moments<S>), grid, block, 0, c->stream, p, buf, PV, buffered);
+            break;
+        default:
+            hipLaunchKernelGGL((k_split_collide<L, S>), grid, block, 0, c->stream, p, buf, PV);
+            LBM_HIP_TRY(hipGetLastError());
+            if (c->two_lattice)
+                c->cur ^= 1;
+            c->iteration++;
+            return LBM_OK;
+        }
+        LBM_HIP_TRY(hipGetLastError());
         return LBM_OK;
     }
 
@@ -563,13 +706,14 @@ struct VTable
     int (*encode)(lbm_ctx *, const double *);
     int (*equilibrium)(lbm_ctx *, const double *, const double *, const double *);
     int (*reduce)(lbm_ctx *, int, double *);
+    int (*split)(lbm_ctx *, int, double *);
 };
 
 template <int L, int S>
 static VTable make_vtable()
 {
     return VTable{&Impl<L, S>::step, &Impl<L, S>::init, &Impl<L, S>::decode, &Impl<L, S>::encode,
-                  &Impl<L, S>::equilibrium, &Impl<L, S>::reduce};
+                  &Impl<L, S>::equilibrium, &Impl<L, S>::reduce, &Impl<L, S>::split};
 }
 
 template <int L>
@@ -693,6 +837,7 @@ void lbm_destroy(lbm_ctx *c)
     (void)hipFree(c->res);
     (void)hipFree(c->uycol);
     (void)hipFree(c->snap);
+    (void)hipFree(c->pass);
     (void)hipFree(c->partials);
     (void)hipFree(c->red_out);
     if (c->ev0)
@@ -883,6 +1028,57 @@ int lbm_run(lbm_ctx *c, uint32_t steps, uint32_t frame_interval)
     float ms = 0.f;
     LBM_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    return LBM_OK;
+}
+
+int lbm_run_split(lbm_ctx *c, uint32_t steps, lbm_pass_callback cb, void *user)
+{
+    if (!c)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_run_split: null context");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_run_split: call lbm_init_populations first");
+    if (!c->pass)
+        LBM_HIP_TRY(hipMalloc(&c->pass, 9ull * c->P * c->Vx * sizeof(double)));
+    const VTable vt = vtable(c);
+    double total_ms = 0.0;
+    for (uint32_t i = 0; i < steps; ++i)
+    {
+        for (int pass = LBM_PASS_STREAM; pass <= LBM_PASS_COLLIDE; ++pass)
+        {
+            LBM_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+            if (int rc = vt.split(c, pass, c->pass))
+                return rc;
+            LBM_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+            LBM_HIP_TRY(hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            LBM_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            total_ms += ms;
+            if (cb)
+                cb(user, pass, c->iteration);
+        }
+    }
+    c->last_ms = total_ms;
+    return LBM_OK;
+}
+
+int lbm_get_pass_populations(lbm_ctx *c, double *values, size_t count)
+{
+    if (!c || !values)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_pass_populations: null argument");
+    if (!c->pass)
+        return fail(LBM_ERR_STATE, "lbm_get_pass_populations: no split step has run");
+    if (count != 9ull * c->N)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_pass_populations: count must be 9 * expanded node count");
+    const size_t PV = size_t(c->P) * c->Vx;
+    std::vector<double> phys(9 * PV);
+    LBM_HIP_TRY(hipMemcpyAsync(phys.data(), c->pass, phys.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint32_t n = 0; n < c->N; ++n)
+    {
+        const size_t pn = c->X0 + n % c->Hx + size_t(n / c->Hx) * c->P;
+        for (int d = 0; d < 9; ++d)
+            values[size_t(d) * c->N + n] = phys[size_t(d) * PV + pn];
+    }
     return LBM_OK;
 }
 

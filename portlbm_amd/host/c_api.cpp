@@ -273,6 +273,29 @@ int portlbm_export_csv(portlbm_handler *h, const char *path)
     });
 }
 
+int portlbm_debug_history(portlbm_handler *h, int field, double *out, size_t count, size_t *needed)
+{
+    return guarded([&] {
+        need(h);
+        need(needed);
+        const std::vector<double> *v = nullptr;
+        switch (field)
+        {
+        case PORTLBM_DENSITY: v = &h->impl->debug_densities(); break;
+        case PORTLBM_X_VELOCITY: v = &h->impl->debug_x_velocities(); break;
+        case PORTLBM_Y_VELOCITY: v = &h->impl->debug_y_velocities(); break;
+        default: throw lbm::exceptions::json::PropertyArgumentException("unknown field");
+        }
+        *needed = v->size();
+        if (out)
+        {
+            if (count != v->size())
+                throw lbm::exceptions::json::PropertyArgumentException("count must equal the history length");
+            std::memcpy(out, v->data(), count * sizeof(double));
+        }
+    });
+}
+
 int portlbm_validate_json(const char *json_text)
 {
     return guarded([&] { parse(json_text).validate(); });

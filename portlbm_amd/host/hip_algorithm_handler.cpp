@@ -6,8 +6,10 @@
 #include <fstream>
 #include <cstdio>
 #include <algorithm>
+#include <iostream>
 #include <string>
 
+#include "lbm/core/access.hpp"
 #include "lbm/core/domain_initialization.hpp"
 #include "lbm/exceptions/exceptions.hpp"
 #include "lbm/file_interaction/file_interaction.hpp"
@@ -161,6 +163,9 @@ void HipAlgorithmHandler::initialize_algorithm()
     check(lbm_sync(ctx_), "lbm_sync");
     fluid_nodes_ = static_cast<std::size_t>(std::count(phase_.begin(), phase_.end(), std::int8_t{0}));
     device_ms_ = 0.0;
+    debug_densities_.clear();
+    debug_x_velocities_.clear();
+    debug_y_velocities_.clear();
     initialization_time = timer.elapsed();
     // a ready future, so that block_until_finished() before start() returns
     // at once, as in the reference (sycl_algorithm.hpp:50-53)
@@ -206,12 +211,101 @@ void HipAlgorithmHandler::execute()
         copy_macroscopic_observables_to_cpu();
 }
 
+// Debug mode (nptl.hpp:165-514 and the lptl/npol/nsol debug drivers): one
+// split step at a time -- stream, outlet, moments, collision as separate
+// launches -- the lattice printed after every pass when it has at most
+// kDebugPrintNodes nodes, and the fields of every iteration kept.
+namespace
+{
+constexpr std::size_t kDebugPrintNodes = 400;
+const char *const kPassNames[4] = {"after streaming", "after the outlet update", "after the moments",
+                                   "after collision"};
+}  // namespace
+
+void HipAlgorithmHandler::pass_callback(void *user, int pass, std::uint64_t iteration)
+{
+    static_cast<HipAlgorithmHandler *>(user)->print_pass(pass, iteration);
+}
+
+void HipAlgorithmHandler::print_pass(int pass, std::uint64_t iteration)
+{
+    const std::size_t N = domain_->total_node_count;
+    if (N > kDebugPrintNodes)
+        return;
+    std::vector<double> values(9 * N);
+    if (pass == LBM_PASS_COLLIDE)
+    {
+        check(lbm_get_populations(ctx_, values.data(), values.size()), "lbm_get_populations");
+        std::cout << "Lattice " << kPassNames[pass] << " (iteration " << iteration << "):\n";
+    }
+    else
+    {
+        check(lbm_get_pass_populations(ctx_, values.data(), values.size()), "lbm_get_pass_populations");
+        std::cout << "Populations " << kPassNames[pass] << " (iteration " << iteration << "):\n";
+    }
+    const unsigned int hx = domain_->horizontal_nodes;
+    char buf[32];
+    for (std::size_t n = 0; n < N; ++n)
+    {
+        if (phase_[n] != 0)
+            continue;
+        std::cout << "  (" << n % hx << "," << n / hx << ")";
+        for (int d = 0; d < 9; ++d)
+        {
+            // collision dumps are in the layout order of lbm_get_populations, pass dumps direction-major
+            std::size_t i = d * N + n;
+            if (pass == LBM_PASS_COLLIDE)
+            {
+                const auto node = static_cast<unsigned int>(n), dir = static_cast<unsigned int>(d);
+                const auto total = static_cast<unsigned int>(N);
+                if (properties_->data_layout == "collision")
+                    i = core::access::CollisionAccessor::at(node, dir, total);
+                else if (properties_->data_layout == "bundle")
+                    i = core::access::BundleAccessor::at(node, dir, total);
+                else
+                    i = core::access::StreamAccessor::at(node, dir, total);
+            }
+            std::snprintf(buf, sizeof buf, " %.6f", values[i]);
+            std::cout << buf;
+        }
+        std::cout << "\n";
+    }
+    std::cout << std::flush;
+}
+
+void HipAlgorithmHandler::execute_debug()
+{
+    const std::size_t D = properties_->domain_node_count;
+    while (control_->is_execution_allowed())
+    {
+        control_->reset_timer();
+        const unsigned int it = control_->get_current_iteration();
+        std::cout << "===== Iteration " << it << " =====\n";
+        check(lbm_run_split(ctx_, 1, &HipAlgorithmHandler::pass_callback, this), "lbm_run_split");
+        copy_macroscopic_observables_to_cpu();
+        debug_densities_.insert(debug_densities_.end(), densities_.begin(), densities_.begin() + D);
+        debug_x_velocities_.insert(debug_x_velocities_.end(), x_velocities_.begin(), x_velocities_.begin() + D);
+        debug_y_velocities_.insert(debug_y_velocities_.end(), y_velocities_.begin(), y_velocities_.begin() + D);
+        double ms = 0.0;
+        check(lbm_last_run_ms(ctx_, &ms), "lbm_last_run_ms");
+        device_ms_ += ms;
+        control_->finalize_iterations(1, control_->elapsed_ms());
+        std::cout << "Finished iteration " << it + 1 << " after " << control_->get_last_frametime()
+                  << " milliseconds.\n";
+    }
+    if (control_->is_finished())
+        std::cout << "All done, exiting simulation.\n" << std::flush;
+}
+
 void HipAlgorithmHandler::start()
 {
     if (!active_)
     {
         control_->allow_execution();
-        future_ = std::async(std::launch::async, [this] { execute(); });
+        if (properties_->debug_mode)
+            future_ = std::async(std::launch::async, [this] { execute_debug(); });
+        else
+            future_ = std::async(std::launch::async, [this] { execute(); });
         active_ = true;
     }
 }
