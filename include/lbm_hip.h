@@ -165,7 +165,8 @@ int lbm_run(lbm_ctx *ctx, uint32_t steps, uint32_t frame_interval);
  * written every step), the collision.  Between passes the populations live in
  * an FP64 pass buffer.  A split step performs the fused step's arithmetic in
  * the same order and equals it bit for bit, for every algorithm, layout and
- * storage. */
+ * storage, on all but solid nodes (whose never-read slots the fused kernels
+ * zero by wave segments). */
 enum lbm_pass { LBM_PASS_STREAM = 0, LBM_PASS_OUTLET = 1, LBM_PASS_MACROSCOPIC = 2, LBM_PASS_COLLIDE = 3 };
 /* Called after each pass with the iteration count (after the collide pass it
  * counts the finished step); the callback may read the pass buffer

@@ -1,7 +1,8 @@
 """Split (debug) steps -- stream, outlet, moments, collision as separate
 launches over an FP64 pass buffer (the reference's debug kernels,
-nptl_kernels.hpp:48-295) -- equal the fused steps bit for bit, and each pass
-does what it says: checked against host arithmetic on the pass buffer."""
+nptl_kernels.hpp:48-295) -- equal the fused steps bit for bit on every fluid,
+ghost and buffer node and in the results, and each pass does what it says:
+checked against host arithmetic on the pass buffer."""
 import numpy as np
 import pytest
 
@@ -28,16 +29,38 @@ def _doc(alg, layout, storage="f64", rounding="nearest", scenario="circle"):
 def test_split_steps_equal_fused_steps(alg, layout, storage, rounding):
     doc = _doc(alg, layout, storage, rounding)
     with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
+        # solid slots are never read; the fused kernels zero some of them by wave segments
+        keep = _not_solid_slots(fused)
         fused.run(37)
         split.run_split(37)
         assert split.iteration() == 37
-        np.testing.assert_array_equal(split.populations(), fused.populations())
+        np.testing.assert_array_equal(split.populations()[keep], fused.populations()[keep])
         for a, b in zip(split.macros(), fused.macros()):
             np.testing.assert_array_equal(a, b)
         # and the two keep agreeing when the fused run continues a split one
         split.run(5)
         fused.run_split(5)
-        np.testing.assert_array_equal(split.populations(), fused.populations())
+        np.testing.assert_array_equal(split.populations()[keep], fused.populations()[keep])
+
+
+def _not_solid_slots(lat):
+    """Population indices (layout order of lbm_get_populations) of every node that is not solid."""
+    N = lat.N
+    ok = lat.phase() != 1
+    n = np.arange(N)
+    layout = lat.desc.layout
+    idx = []
+    for d in range(9):
+        if layout == pa.LAYOUTS["collision"]:
+            idx.append(9 * n + d)
+        elif layout == pa.LAYOUTS["stream"]:
+            idx.append(d * N + n)
+        else:
+            idx.append(3 * (d // 3) * N + d % 3 + 3 * n)
+    mask = np.zeros(9 * N, bool)
+    for d in range(9):
+        mask[idx[d][ok]] = True
+    return mask
 
 
 def test_each_pass_matches_host_arithmetic():
