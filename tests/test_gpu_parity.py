@@ -63,3 +63,41 @@ def test_scenarios(alg, scenario):
     r = compare(doc, 400)
     print(alg, scenario, r, flush=True)
     assert r["ok"], r
+
+
+# nptl runs an LDS-staged kernel in some layout x storage combinations
+# (collision in FP32/FP16/BF16, bundle in FP64/FP32); lptl runs the plain pull
+# kernel with the same arithmetic, so their fluid fields agree bit for bit.
+@pytest.mark.parametrize("layout,storage,rounding", [
+    ("collision", "f32", "nearest"), ("collision", "f16-shifted", "nearest"),
+    ("collision", "f16-shifted", "stochastic"), ("collision", "bf16-shifted", "nearest"),
+    ("bundle", "f64", "nearest"), ("bundle", "f32", "nearest"), ("collision", "f64", "nearest"),
+    ("bundle", "f16-shifted", "nearest")])
+@pytest.mark.parametrize("scenario,h,v", [("circle", 130, 40), ("porous-circles", 200, 64), ("walls", 3, 3)])
+def test_nptl_layouts_equal_lptl_in_every_storage(layout, storage, rounding, scenario, h, v):
+    def doc(alg):
+        return pa.settings(alg, layout, inner_h=h, inner_v=v, scenario=scenario, work_group_size=256, tau=0.7,
+                           rho_in=1.05, storage=storage, rounding=rounding, radiusMin=2.0, radiusMax=6.0,
+                           obstacleMargin=4)
+    with pa.Lattice.from_settings(doc("nptl")) as a, pa.Lattice.from_settings(doc("lptl")) as b:
+        a.run(57)
+        b.run(57)
+        for x, y in zip(a.macros(), b.macros()):
+            np.testing.assert_array_equal(x, y)
+        # fluid populations, node by node (nptl may expand the lattice; coordinates agree)
+        fa, fb = a.populations(), b.populations()
+        hxa, hxb = a.desc.expanded_horizontal_nodes, b.desc.expanded_horizontal_nodes
+        pb = b.phase()
+        nb = np.flatnonzero(pb == 0)
+        xs, ys = nb % hxb, nb // hxb
+        na = xs + ys * hxa
+        assert np.all(a.phase()[na] == 0)
+        d = np.arange(9)[None, :]
+        def idx(n, N):
+            n = n[:, None]
+            if layout == "collision":
+                return 9 * n + d
+            if layout == "stream":
+                return N * d + n
+            return 3 * (d // 3) * N + d % 3 + 3 * n
+        np.testing.assert_array_equal(fa[idx(na, a.N)], fb[idx(nb, b.N)])
