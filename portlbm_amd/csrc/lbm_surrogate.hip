@@ -12,7 +12,8 @@
 //                  the same kernel on flipped, transposed weights.
 //   k_conv_wgrad   16 x 16 (out, in) channel pairs per block, each thread one
 //                  pair's 9 taps + bias over a sliding 3x3 register window;
-//                  per-slice partials reduced in a fixed order (deterministic).
+//                  per-slice partials summed in two fixed-order stages
+//                  (32 slice groups in parallel, then the groups).
 //   pooling, upsampling, ReLU mask, loss and Adam: one thread per element.
 #include <hip/hip_runtime.h>
 
@@ -193,34 +194,62 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float *__restrict__ x,
         out[k] = acc[k];
 }
 
-// gw[o][c][k] = sum over slices (fixed order, FP64); gb[o] from channel 0's slot 9.
-__global__ void k_wgrad_reduce(const float *__restrict__ partial, float *__restrict__ gw, float *__restrict__ gb,
-                               int Cin, int Cout, int CinP, int CoutP, int slices)
+// Weight-gradient sums over the slices in two stages, each in a fixed order
+// (bitwise reproducible): stage 1 sums group g's slices [g*per, (g+1)*per)
+// of one weight per thread into part2[g][weight] (FP64); stage 2 adds the
+// groups.  Weight index w < Cout*Cin*9 is (o, c, k), then Cout biases (slot 9
+// of channel 0).
+constexpr int kReduceGroups = 32;
+
+__device__ __forceinline__ size_t wgrad_slot(int w, int Cin, int Cout, int CinP)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int nw = Cout * Cin * 9;
-    if (idx >= nw + Cout)
-        return;
     int o, c, k;
-    if (idx < nw)
+    if (w < nw)
     {
-        o = idx / (Cin * 9);
-        c = (idx / 9) % Cin;
-        k = idx % 9;
+        o = w / (Cin * 9);
+        c = (w / 9) % Cin;
+        k = w % 9;
     }
     else
     {
-        o = idx - nw;
+        o = w - nw;
         c = 0;
         k = 9;
     }
+    return (size_t(o) * CinP + c) * 10 + k;
+}
+
+__global__ void k_wgrad_reduce(const float *__restrict__ partial, double *__restrict__ part2, int Cin, int Cout,
+                               int CinP, int CoutP, int slices, int per)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nall = Cout * Cin * 9 + Cout;
+    if (w >= nall)
+        return;
+    const size_t slot = wgrad_slot(w, Cin, Cout, CinP);
+    const size_t stride = size_t(CoutP) * CinP * 10;
+    const int s0 = blockIdx.y * per, s1 = min(slices, s0 + per);
     double s = 0.0;
-    for (int sl = 0; sl < slices; ++sl)
-        s += double(partial[((size_t(sl) * CoutP + o) * CinP + c) * 10 + k]);
-    if (idx < nw)
-        gw[idx] = float(s);
+    for (int sl = s0; sl < s1; ++sl)
+        s += double(partial[size_t(sl) * stride + slot]);
+    part2[size_t(blockIdx.y) * nall + w] = s;
+}
+
+__global__ void k_wgrad_final(const double *__restrict__ part2, float *__restrict__ gw, float *__restrict__ gb,
+                              int Cin, int Cout, int groups)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nw = Cout * Cin * 9, nall = nw + Cout;
+    if (w >= nall)
+        return;
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g)
+        s += part2[size_t(g) * nall + w];
+    if (w < nw)
+        gw[w] = float(s);
     else
-        gb[o] = float(s);
+        gb[w - nw] = float(s);
 }
 
 // wp[(c*9 + k)*CoutP + o] = w[o][c][k];  wt[(o*9 + 8-k)*CinP + c] = w[o][c][k]
@@ -391,6 +420,7 @@ struct lbm_surrogate
     float *a[14] = {}, *p[5] = {}, *z[4] = {}, *da[14] = {};
     float *partial = nullptr;
     size_t partial_floats = 0;
+    double *part2 = nullptr;  // kReduceGroups x (weights + biases) of the largest conv
     double *loss_part = nullptr, *loss_dev = nullptr;
     float *history = nullptr;
     size_t history_cap = 0;
@@ -481,8 +511,11 @@ void conv_wgrad(lbm_surrogate *s, const float *x, const float *g, int k, int l, 
     hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, s->stream, x, g, s->partial, B, kCin[k], kCout[k], CinP,
                        CoutP, s->LH[l], s->LW[l], tiles_x(s, l), tiles(s, l), slices);
     const int n = kCout[k] * kCin[k] * 9 + kCout[k];
-    hipLaunchKernelGGL(k_wgrad_reduce, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->partial,
-                       s->grad + s->w_off[k], s->grad + s->b_off[k], kCin[k], kCout[k], CinP, CoutP, slices);
+    const int groups = std::min(kReduceGroups, slices), per = (slices + groups - 1) / groups;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((n + 255) / 256, groups), dim3(256), 0, s->stream, s->partial,
+                       s->part2, kCin[k], kCout[k], CinP, CoutP, slices, per);
+    hipLaunchKernelGGL(k_wgrad_final, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->part2,
+                       s->grad + s->w_off[k], s->grad + s->b_off[k], kCin[k], kCout[k], groups);
 }
 
 void pool(lbm_surrogate *s, const float *x, float *y, int C, int l_in, int B)
@@ -714,6 +747,14 @@ int lbm_surrogate_create(uint32_t height, uint32_t width, uint32_t max_batch, lb
     }
     s->partial_floats = partial;
     C_TRY(alloc(s, &s->partial, partial));
+    {
+        size_t most = 0;
+        for (int k = 0; k < kConvs; ++k)
+            most = std::max(most, size_t(kCout[k]) * kCin[k] * 9 + kCout[k]);
+        float *p2 = nullptr;
+        C_TRY(alloc(s, &p2, 2 * kReduceGroups * most));  // doubles
+        s->part2 = reinterpret_cast<double *>(p2);
+    }
     C_TRY(alloc(s, &s->in, act_floats(s, LBM_SURROGATE_INPUTS, 0)));
     C_TRY(alloc(s, &s->tgt, act_floats(s, LBM_SURROGATE_OUTPUTS, 0)));
     C_TRY(alloc(s, &s->out, act_floats(s, LBM_SURROGATE_OUTPUTS, 0)));
