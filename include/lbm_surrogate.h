@@ -51,6 +51,10 @@ int lbm_surrogate_get_params(lbm_surrogate *s, float *params, size_t count);
 /* Forward pass: inputs (batch x 4 x H x W) -> outputs (batch x 3 x H x W),
  * host arrays; any batch size (processed in chunks of max_batch). */
 int lbm_surrogate_predict(lbm_surrogate *s, const float *inputs, uint32_t batch, float *outputs);
+/* The same with the convolutions on the matrix cores in BF16 (inputs of each
+ * convolution and its weights rounded to BF16, FP32 accumulation; pooling,
+ * upsampling and the skips in FP32). */
+int lbm_surrogate_predict_bf16(lbm_surrogate *s, const float *inputs, uint32_t batch, float *outputs);
 
 /* Loss and the gradient of every parameter for one host batch (batch <= max_batch). */
 int lbm_surrogate_loss_grad(lbm_surrogate *s, const float *inputs, const float *targets, uint32_t batch,

@@ -252,6 +252,119 @@ __global__ void k_wgrad_final(const double *__restrict__ part2, float *__restric
         gb[w - nw] = float(s);
 }
 
+// ---- BF16 inference: implicit-GEMM convolution on the matrix cores --------
+// out[o][p] = sum_k Wb[o][k] X[k][p], k = c*9 + tap (the FP32 weight order),
+// with v_mfma_f32_16x16x16bf16_1k (FP32 accumulation).  Lane layout of that
+// instruction, checked on the device: A[i][k] in lane i + 16 (k/4), element
+// k%4; B[k][j] in lane j + 16 (k/4), element k%4; D[i][j] in lane
+// j + 16 (i/4), element i%4.  A 256-thread block computes 16 pixels of 4 rows
+// (one row per wave) for all output channels: the input tile with its halo
+// is staged in LDS as BF16 (round to nearest even), each wave builds the
+// im2col fragment of a 16-wide K chunk once and reuses it for every 16-channel
+// output tile; weights come packed as BF16 [Cout16][K16] (zero padded).
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaRows = 4;  // output rows per block (one per wave)
+constexpr int kMfmaCols = 16;
+constexpr int kMaxCoTiles = 4;  // Cout <= 64
+
+__device__ __forceinline__ short f32_to_bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        return short((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return short(u >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_conv3x3_bf16(const float *__restrict__ x, const short *__restrict__ wb,
+                                                      const float *__restrict__ bias, float *__restrict__ y, int Cin,
+                                                      int Cout, int Kp, int H, int W, int tiles_x, int relu)
+{
+    constexpr int TR = kMfmaRows + 2, TC = kMfmaCols + 2;
+    extern __shared__ short tile_bf[];  // Cin x TR x TC
+    const int b = blockIdx.y;
+    const int x0 = int(blockIdx.x % unsigned(tiles_x)) * kMfmaCols;
+    const int y0 = int(blockIdx.x / unsigned(tiles_x)) * kMfmaRows;
+    const float *xb = x + size_t(b) * Cin * H * W;
+    for (int i = threadIdx.x; i < Cin * TR * TC; i += 256)
+    {
+        const int c = i / (TR * TC), r = (i / TC) % TR, col = i % TC;
+        const int yy = y0 - 1 + r, xx = x0 - 1 + col;
+        float v = 0.f;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+            v = xb[(size_t(c) * H + yy) * W + xx];
+        tile_bf[i] = f32_to_bf16_rne(v);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int j = lane % 16, kq = 4 * (lane / 16);
+    const int co_tiles = (Cout + 15) / 16;
+    const int K = 9 * Cin;
+    f32x4 acc[kMaxCoTiles];
+#pragma unroll
+    for (int t = 0; t < kMaxCoTiles; ++t)
+        acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < Kp; kc += 16)
+    {
+        bf16x4 bv;
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+        {
+            const int k = kc + kq + it;
+            short v = 0;
+            if (k < K)
+            {
+                const int c = k / 9, tap = k - 9 * (k / 9);
+                v = tile_bf[(c * TR + wave + tap / 3) * TC + j + tap % 3];
+            }
+            bv[it] = v;
+        }
+#pragma unroll
+        for (int t = 0; t < kMaxCoTiles; ++t)
+        {
+            if (t < co_tiles)
+            {
+                const bf16x4 av = *reinterpret_cast<const bf16x4 *>(wb + size_t(t * 16 + j) * Kp + kc + kq);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    const int oy = y0 + wave, ox = x0 + j;
+    if (oy >= H || ox >= W)
+        return;
+#pragma unroll
+    for (int t = 0; t < kMaxCoTiles; ++t)
+    {
+        if (t >= co_tiles)
+            break;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const int co = t * 16 + kq + v;
+            if (co < Cout)
+            {
+                float r = acc[t][v] + bias[co];
+                if (relu)
+                    r = r > 0.f ? r : 0.f;
+                y[((size_t(b) * Cout + co) * H + oy) * W + ox] = r;
+            }
+        }
+    }
+}
+
+// wb[o*Kp + k] = bf16(w[o*9Cin + k]) for k < 9 Cin (padding stays zero)
+__global__ void k_pack_bf16(const float *__restrict__ w, short *__restrict__ wb, int Cin, int Cout, int Kp)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = 9 * Cin;
+    if (idx >= Cout * K)
+        return;
+    const int o = idx / K, k = idx % K;
+    wb[size_t(o) * Kp + k] = f32_to_bf16_rne(w[idx]);
+}
+
 // wp[(c*9 + k)*CoutP + o] = w[o][c][k];  wt[(o*9 + 8-k)*CinP + c] = w[o][c][k]
 __global__ void k_pack(const float *__restrict__ w, float *__restrict__ wp, float *__restrict__ wt, int Cin, int Cout,
                        int CinP, int CoutP)
@@ -414,6 +527,7 @@ struct lbm_surrogate
     size_t w_off[kConvs] = {}, b_off[kConvs] = {};
     float *params = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
     float *wp[kConvs] = {}, *wt[kConvs] = {};
+    short *wb[kConvs] = {};  // BF16 [Cout16][K16] weights of the matrix-core inference
     bool packed = false;
     uint32_t adam_t = 0;
     float *in = nullptr, *tgt = nullptr, *out = nullptr, *dout = nullptr, *dtmp = nullptr;
@@ -482,14 +596,30 @@ int pack_all(lbm_surrogate *s)
         const int n = kCout[k] * kCin[k] * 9;
         hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->params + s->w_off[k], s->wp[k],
                            s->wt[k], kCin[k], kCout[k], round16(kCin[k]), round16(kCout[k]));
+        hipLaunchKernelGGL(k_pack_bf16, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->params + s->w_off[k],
+                           s->wb[k], kCin[k], kCout[k], round16(9 * kCin[k]));
     }
     S_TRY(hipGetLastError());
     s->packed = true;
     return LBM_OK;
 }
 
-void conv_forward(lbm_surrogate *s, const float *x, int k, float *y, int l, int B, bool relu)
+void conv_forward_bf16(lbm_surrogate *s, const float *x, int k, float *y, int l, int B, bool relu)
 {
+    const int tx = (s->LW[l] + kMfmaCols - 1) / kMfmaCols, ty = (s->LH[l] + kMfmaRows - 1) / kMfmaRows;
+    const size_t lds = size_t(kCin[k]) * (kMfmaRows + 2) * (kMfmaCols + 2) * sizeof(short);
+    hipLaunchKernelGGL(k_conv3x3_bf16, dim3(unsigned(tx * ty), unsigned(B)), dim3(256), lds, s->stream, x, s->wb[k],
+                       s->params + s->b_off[k], y, kCin[k], kCout[k], round16(9 * kCin[k]), s->LH[l], s->LW[l], tx,
+                       relu ? 1 : 0);
+}
+
+void conv_forward(lbm_surrogate *s, const float *x, int k, float *y, int l, int B, bool relu, bool bf16 = false)
+{
+    if (bf16)
+    {
+        conv_forward_bf16(s, x, k, y, l, B, relu);
+        return;
+    }
     const dim3 grid(unsigned(tiles(s, l)), unsigned(round16(kCout[k]) / kCoT), unsigned(B));
     hipLaunchKernelGGL(k_conv3x3, grid, dim3(256), 0, s->stream, x, s->wp[k], s->params + s->b_off[k], y, kCin[k],
                        kCout[k], round16(kCout[k]), s->LH[l], s->LW[l], tiles_x(s, l), relu ? 1 : 0);
@@ -550,26 +680,27 @@ void relu_mask(lbm_surrogate *s, float *g, const float *a, int C, int l, int B)
     hipLaunchKernelGGL(k_relu_mask, dim3(elem_grid(n)), dim3(256), 0, s->stream, g, a, n);
 }
 
-// Forward over s->in (B samples) into s->a, s->p, s->z, s->out.
-void forward(lbm_surrogate *s, int B)
+// Forward over s->in (B samples) into s->a, s->p, s->z, s->out; bf16: the
+// convolutions on the matrix cores (inference).
+void forward(lbm_surrogate *s, int B, bool bf16 = false)
 {
-    conv_forward(s, s->in, 0, s->a[1], 0, B, true);
-    conv_forward(s, s->a[1], 1, s->a[2], 0, B, true);
+    conv_forward(s, s->in, 0, s->a[1], 0, B, true, bf16);
+    conv_forward(s, s->a[1], 1, s->a[2], 0, B, true, bf16);
     for (int l = 1; l <= 3; ++l)
     {
         pool(s, s->a[2 * l], s->p[l], kCout[2 * l - 1], l - 1, B);
-        conv_forward(s, s->p[l], 2 * l, s->a[2 * l + 1], l, B, true);
-        conv_forward(s, s->a[2 * l + 1], 2 * l + 1, s->a[2 * l + 2], l, B, true);
+        conv_forward(s, s->p[l], 2 * l, s->a[2 * l + 1], l, B, true, bf16);
+        conv_forward(s, s->a[2 * l + 1], 2 * l + 1, s->a[2 * l + 2], l, B, true, bf16);
     }
     pool(s, s->a[8], s->p[4], kCout[7], 3, B);
-    conv_forward(s, s->p[4], 8, s->a[9], 4, B, true);
+    conv_forward(s, s->p[4], 8, s->a[9], 4, B, true, bf16);
     for (int m = 0; m < 4; ++m)
     {
         const int l = 3 - m, kk = 9 + m;
         upadd(s, s->a[9 + m], s->a[8 - 2 * m], s->z[l], kCin[kk], l, B);
-        conv_forward(s, s->z[l], kk, s->a[10 + m], l, B, true);
+        conv_forward(s, s->z[l], kk, s->a[10 + m], l, B, true, bf16);
     }
-    conv_forward(s, s->a[13], 13, s->out, 0, B, false);
+    conv_forward(s, s->a[13], 13, s->out, 0, B, false, bf16);
 }
 
 // Loss of s->out against s->tgt and the full parameter gradient into s->grad.
@@ -738,9 +869,14 @@ int lbm_surrogate_create(uint32_t height, uint32_t width, uint32_t max_batch, lb
         const size_t cin_p = size_t(round16(kCin[k])), cout_p = size_t(round16(kCout[k]));
         C_TRY(alloc(s, &s->wp[k], cin_p * 9 * cout_p));
         C_TRY(alloc(s, &s->wt[k], cin_p * 9 * cout_p));
+        const size_t kp = size_t(round16(9 * kCin[k]));
+        float *wbf = nullptr;
+        C_TRY(alloc(s, &wbf, (cout_p * kp + 1) / 2));
+        s->wb[k] = reinterpret_cast<short *>(wbf);
         C_TRY([&]() -> int {
             S_TRY(hipMemsetAsync(s->wp[k], 0, cin_p * 9 * cout_p * sizeof(float), s->stream));
             S_TRY(hipMemsetAsync(s->wt[k], 0, cin_p * 9 * cout_p * sizeof(float), s->stream));
+            S_TRY(hipMemsetAsync(s->wb[k], 0, cout_p * kp * sizeof(short), s->stream));
             return LBM_OK;
         }());
         partial = std::max(partial, size_t(wgrad_slices(s, k, conv_level(k), s->Bmax)) * cout_p * cin_p * 10);
@@ -834,7 +970,7 @@ int lbm_surrogate_get_params(lbm_surrogate *s, float *params, size_t count)
     return LBM_OK;
 }
 
-int lbm_surrogate_predict(lbm_surrogate *s, const float *inputs, uint32_t batch, float *outputs)
+static int predict(lbm_surrogate *s, const float *inputs, uint32_t batch, float *outputs, bool bf16)
 {
     if (int rc = check_ready(s, "lbm_surrogate_predict"))
         return rc;
@@ -847,7 +983,7 @@ int lbm_surrogate_predict(lbm_surrogate *s, const float *inputs, uint32_t batch,
         const int B = int(std::min<uint32_t>(uint32_t(s->Bmax), batch - b0));
         S_TRY(hipMemcpyAsync(s->in, inputs + size_t(b0) * LBM_SURROGATE_INPUTS * hw,
                              size_t(B) * LBM_SURROGATE_INPUTS * hw * sizeof(float), hipMemcpyHostToDevice, s->stream));
-        forward(s, B);
+        forward(s, B, bf16);
         S_TRY(hipGetLastError());
         S_TRY(hipMemcpyAsync(outputs + size_t(b0) * LBM_SURROGATE_OUTPUTS * hw, s->out,
                              size_t(B) * LBM_SURROGATE_OUTPUTS * hw * sizeof(float), hipMemcpyDeviceToHost,
@@ -859,6 +995,16 @@ int lbm_surrogate_predict(lbm_surrogate *s, const float *inputs, uint32_t batch,
     S_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;
     return LBM_OK;
+}
+
+int lbm_surrogate_predict(lbm_surrogate *s, const float *inputs, uint32_t batch, float *outputs)
+{
+    return predict(s, inputs, batch, outputs, false);
+}
+
+int lbm_surrogate_predict_bf16(lbm_surrogate *s, const float *inputs, uint32_t batch, float *outputs)
+{
+    return predict(s, inputs, batch, outputs, true);
 }
 
 static int host_batch(lbm_surrogate *s, const float *inputs, const float *targets, uint32_t batch, double *norm,

@@ -59,6 +59,7 @@ def _lib() -> ctypes.CDLL:
             "lbm_surrogate_set_params": ([_P, _P, _SZ], ctypes.c_int),
             "lbm_surrogate_get_params": ([_P, _P, _SZ], ctypes.c_int),
             "lbm_surrogate_predict": ([_P, _P, _U32, _P], ctypes.c_int),
+            "lbm_surrogate_predict_bf16": ([_P, _P, _U32, _P], ctypes.c_int),
             "lbm_surrogate_loss_grad": ([_P, _P, _P, _U32, ctypes.POINTER(ctypes.c_double), _P, _SZ], ctypes.c_int),
             "lbm_surrogate_train_step": ([_P, _P, _P, _U32, ctypes.c_float, ctypes.POINTER(ctypes.c_double)],
                                          ctypes.c_int),
@@ -129,11 +130,13 @@ class Surrogate:
         if a.ndim != 4 or a.shape[1:] != (channels, self.height, self.width):
             raise ValueError(f"expected (B, {channels}, {self.height}, {self.width}), got {a.shape}")
 
-    def predict(self, inputs: np.ndarray) -> np.ndarray:
+    def predict(self, inputs: np.ndarray, bf16: bool = False) -> np.ndarray:
+        """Forward pass; bf16: convolutions on the matrix cores in BF16 (FP32 accumulation)."""
         x = _f32(inputs)
         self._check_shape(x, INPUTS)
         out = np.empty((x.shape[0], OUTPUTS, self.height, self.width), dtype=np.float32)
-        _check_hip(self.lib.lbm_surrogate_predict(self._s, _ptr(x), x.shape[0], _ptr(out)), "lbm_surrogate_predict")
+        fn = self.lib.lbm_surrogate_predict_bf16 if bf16 else self.lib.lbm_surrogate_predict
+        _check_hip(fn(self._s, _ptr(x), x.shape[0], _ptr(out)), "lbm_surrogate_predict")
         return out
 
     def loss_grad(self, inputs: np.ndarray, targets: np.ndarray) -> tuple[float, np.ndarray]:
@@ -272,7 +275,8 @@ class Model:
     warm-start options chosen on validation domains (j_blend, rho_sigma)."""
 
     def __init__(self, params: np.ndarray, height: int, width: int, j_scale: float, rho_scale: float = 1.0,
-                 j_blend: float = 1.0, rho_sigma: float = 0.0, meta: dict | None = None):
+                 j_blend: float = 1.0, rho_sigma: float = 0.0, meta: dict | None = None, bf16: bool = True):
+        self.bf16 = bf16  # warm starts infer with BF16 matrix-core convolutions
         self.params = _f32(params)
         self.height, self.width = int(height), int(width)
         self.j_scale, self.rho_scale = float(j_scale), float(rho_scale)
@@ -377,7 +381,7 @@ def predict(model: Model, docs: list[dict], net: Surrogate | None = None):
     net = net or model.network()
     try:
         masks = [fluid_mask(d) for d in docs]
-        out = net.predict(np.stack([features(m, *_physical(d)) for m, d in zip(masks, docs)]))
+        out = net.predict(np.stack([features(m, *_physical(d)) for m, d in zip(masks, docs)]), bf16=model.bf16)
         return out, masks, net.last_ms()
     finally:
         if own:

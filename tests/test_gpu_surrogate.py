@@ -134,3 +134,54 @@ def test_rejects_bad_arguments():
         net.dataset_reserve(2)
         with pytest.raises(pa.LbmError):
             net.dataset_add(np.ones((3, 4, 16, 16), np.float32), np.zeros((3, 3, 16, 16), np.float32))
+
+
+def _bf16(a):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _torch_bf16_network(params, x):
+    """The network with each convolution's input and weights rounded to BF16 and
+    exact (FP64) accumulation -- what the matrix-core inference computes up to
+    its FP32 accumulation."""
+    import torch
+    import torch.nn.functional as F
+    ws, off = [], 0
+    for ci, co in nn.conv_shapes():
+        w = torch.from_numpy(_bf16(params[off:off + co * ci * 9]).reshape(co, ci, 3, 3))
+        off += co * ci * 9
+        ws.append((w, torch.from_numpy(params[off:off + co].astype(np.float64))))
+        off += co
+
+    def c(k, t, relu=True):
+        y = F.conv2d(torch.from_numpy(_bf16(t.numpy())), ws[k][0], ws[k][1], padding=1)
+        return F.relu(y) if relu else y
+    up = lambda t: F.interpolate(t, scale_factor=2, mode="nearest")  # noqa: E731
+    xt = torch.from_numpy(x.astype(np.float64))
+    a2 = c(1, c(0, xt))
+    a4 = c(3, c(2, F.avg_pool2d(a2, 2)))
+    a6 = c(5, c(4, F.avg_pool2d(a4, 2)))
+    a8 = c(7, c(6, F.avg_pool2d(a6, 2)))
+    a9 = c(8, F.avg_pool2d(a8, 2))
+    a10 = c(9, up(a9) + a8)
+    a11 = c(10, up(a10) + a6)
+    a12 = c(11, up(a11) + a4)
+    a13 = c(12, up(a12) + a2)
+    return c(13, a13, relu=False).numpy()
+
+
+@pytest.mark.parametrize("B,H,W", [(2, 32, 48), (1, 64, 80)])
+def test_bf16_matrix_core_inference(B, H, W):
+    params, x, _ = _case(41 + H, B, H, W)
+    with sg.Surrogate(H, W, max_batch=B) as net:
+        net.set_params(params)
+        out = net.predict(x, bf16=True)
+        ref32 = net.predict(x)
+    emul = _torch_bf16_network(params, x)
+    # against the BF16 emulation: only accumulation order and FP32 rounding differ
+    assert _normwise(out, emul) <= 2e-4
+    # against FP32 inference: the BF16 rounding of 14 layers' inputs and weights
+    assert _normwise(out, ref32) <= 3e-2
