@@ -182,6 +182,65 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: 
     Path(out_path).write_text(json.dumps({"workload": workload, "steps": steps, "rows": rows}, indent=1))
 
 
+def steady_precision_study(out_path: str, inner_h: int = 1024, inner_v: int = 256, taus=(0.6, 0.8),
+                           tol: float = 1e-10, max_steps: int = 400000, block: int = 1000) -> None:
+    """Reduced-precision storage against FP64 at the steady state of a porous
+    channel (porous-circles, radii U[4,16), porosity 0.7, seed 20261202,
+    inlet 1.05 / outlet 1.0, inner_h x inner_v nodes).  FP64 runs in blocks
+    of `block` steps until the residual over the last 100 steps of a block,
+    ||j(t) - j(t-100)|| / ||j(t)||, falls below `tol`; every other storage then
+    runs the same number of steps.  Errors as in precision_study, plus each
+    run's final residual and the first block end at which its residual was
+    below 1e-6."""
+    rows = []
+    for tau in taus:
+        ref = None
+        total = None
+        for variant in ("f64", "f32", "f16-shifted", "f16-shifted/stochastic", "bf16-shifted",
+                        "bf16-shifted/stochastic"):
+            storage, _, rounding = variant.partition("/")
+            doc = workloads.porous_channel(storage=storage, steps=max_steps, tau=tau, inner_h=inner_h,
+                                           inner_v=inner_v)
+            doc["mi355x"]["rounding"] = rounding or "nearest"
+            with pa.Lattice.from_settings(doc) as lat:
+                m0 = lat.reduce()["mass"]
+                steps, res, first_1e6, ms = 0, 1.0, None, 0.0
+                while (steps < total) if total is not None else (res >= tol and steps < max_steps):
+                    ms += lat.run(block - 100, 0)
+                    lat.residual()
+                    ms += lat.run(100, 0)
+                    res = lat.residual()
+                    steps += block
+                    if first_1e6 is None and res < 1e-6:
+                        first_1e6 = steps
+                red = lat.reduce()
+                rho, jx, jy, _ = lat.macros()
+            fluid = rho > -1
+            row = {"tau": tau, "storage": storage, "rounding": rounding or "nearest", "steps": steps,
+                   "fluid_nodes": int(fluid.sum()), "residual": res, "steps_to_1e-6": first_1e6,
+                   "mflups": round(int(fluid.sum()) * steps / (ms * 1e-3) / 1e6, 1),
+                   "mass_drift": red["mass"] / m0 - 1.0, "drag_x": red["fx"]}
+            if storage == "f64":
+                ref, total = (rho, jx, jy, red), steps
+            else:
+                r64, jx64, jy64, red64 = ref
+                f = fluid & (r64 > -1)
+                dj = np.hypot(jx[f] - jx64[f], jy[f] - jy64[f])
+                j64 = np.hypot(jx64[f], jy64[f])
+                row.update({
+                    "rho_l2": float(np.linalg.norm(rho[f] - r64[f]) / np.linalg.norm(r64[f])),
+                    "rho_linf": float(np.abs(rho[f] - r64[f]).max() / np.abs(r64[f]).max()),
+                    "u_l2": float(np.linalg.norm(dj) / np.linalg.norm(j64)),
+                    "u_linf": float(dj.max() / j64.max()),
+                    "drag_x_rel": float(abs(red["fx"] - red64["fx"]) / abs(red64["fx"])),
+                })
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+    workload = (f"porous-circles {inner_h}x{inner_v} (seed 20261202, radii U[4,16), porosity 0.7), "
+                f"rho_in 1.05 / rho_out 1.0, run to the FP64 steady state (residual < {tol:g})")
+    Path(out_path).write_text(json.dumps({"workload": workload, "rows": rows}, indent=1))
+
+
 def batched(out_path: str, steps: int, count: int = 512) -> dict:
     """512 uneven porous channels in FP32 (portlbm_amd.workloads.batch_channels):
     per-domain launches vs one persistent launch per step vs the whole run in
@@ -227,6 +286,8 @@ def main() -> int:
     ap.add_argument("--study-scenario", default="cylinder", choices=("cylinder", "porous"))
     ap.add_argument("--study-taus", default="0.52,0.6,0.8")
     ap.add_argument("--study-steps", type=int, default=20000)
+    ap.add_argument("--steady-study", metavar="OUT.json",
+                    help="reduced-precision storage against FP64 at the steady state of a 1024x256 porous channel")
     ap.add_argument("--batched", metavar="OUT.json")
     ap.add_argument("--batch-steps", type=int, default=2000)
     args = ap.parse_args()
@@ -241,6 +302,8 @@ def main() -> int:
     if args.precision_study:
         taus = tuple(float(t) for t in args.study_taus.split(","))
         precision_study(args.precision_study, args.study_steps, taus, args.study_scenario)
+    if args.steady_study:
+        steady_precision_study(args.steady_study)
     if args.batched:
         batched(args.batched, args.batch_steps)
 
