@@ -153,30 +153,36 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
 // nptl for the collision and bundle layouts.  There a pull instruction of a
 // wave reads every 9th (3rd) element, so each touches nine (three) times the
 // cache lines it uses.  Here the block first copies, with aligned 16-byte
-// loads, the three source ranges its row segment pulls from into LDS:
-//   collision: range r = the nine populations of row y + r - 1 (places x0-1 ..
-//              x0+len), pulls of cy come from range 1 - cy;
-//   bundle:    range r = array r (directions 3r..3r+2, cy = r - 1) of row y - cy,
-//              the only row those directions are pulled from.
+// loads, the source ranges its row segments pull from into LDS:
+//   collision: R node rows y0 .. y0+R-1 per block; range r = the nine
+//              populations of row y0 - 1 + r (places x0-1 .. x0+len), R + 2
+//              ranges, and row ty pulls direction d from range ty + 1 - cy.
+//              Each staged row serves up to three node rows of the block, so
+//              a row is fetched (R + 2) / R times instead of three times;
+//   bundle:    one node row y per block; range r = array r (directions
+//              3r..3r+2, cy = r - 1) of row y - cy, the only row those
+//              directions are pulled from.
 // Bounce-back pulls take the node's own f[8-d]: in the collision layout from
-// range 1; in the bundle layout from range 1 when 8-d is in array 1, else
-// straight from HBM (nodes next to a wall only).  The results go through LDS
-// as well and leave as contiguous 16-byte stores: the segment's nine (three)
-// arrays of the block are written whole (solid nodes as zeros).  A block that
-// holds a node that is neither fluid nor solid (the walls stencil's value 2
-// on tiny grids) writes node by node instead and leaves that node untouched.
-// Used where it measured faster on the 8192 x 2048 porous channel (64-node
-// segments; GFLUPS before -> after): FP32 collision 25.5 -> 34.0, FP16
-// collision 43.6 -> 62.7, FP64 bundle 24.2 -> 28.1, FP32 bundle 42.8 -> 48.8.
-// FP64 collision (its 37 KB of LDS per block) and the 16-bit bundle layout
-// keep the plain kernel.
+// the node's own range; in the bundle layout from range 1 when 8-d is in
+// array 1, else straight from HBM (nodes next to a wall only).  The results
+// go through LDS as well and leave as contiguous 16-byte stores: each node
+// row's nine-population array (collision) or the row's three arrays (bundle)
+// are written whole (solid nodes as zeros).  A block that holds a node that
+// is neither fluid nor solid (the walls stencil's value 2 on tiny grids)
+// writes node by node instead and leaves that node untouched.
+// Used where it measured faster than the plain kernel on the 8192 x 2048
+// porous channel (64-node segments; GFLUPS plain -> staged): FP64 collision
+// 16.3 -> 21.2 (R = 8; R = 2, 4: 18.3, 20.3), FP32 collision 25.5 -> 45.7
+// (R = 4; R = 1, 2, 8: 34.0, 41.0, 34.6), FP16 collision 43.6 -> 64.9 (R = 2;
+// R = 1, 4, 8: 62.7, 53.3, 38.7), FP64 bundle 24.2 -> 28.1, FP32 bundle
+// 42.8 -> 48.8.  The 16-bit bundle layout keeps the plain kernel.
 constexpr uint32_t NPTL_STAGED_BLOCK = 64;
 
 template <int L, int S>
 constexpr bool nptl_staged()
 {
     if constexpr (L == LBM_LAYOUT_COLLISION)
-        return S != LBM_STORE_F64;
+        return true;
     else if constexpr (L == LBM_LAYOUT_BUNDLE)
         return sizeof(typename Storage<S>::T) >= 4;
     else
@@ -184,26 +190,45 @@ constexpr bool nptl_staged()
 }
 
 template <int L, int S>
-__global__ __launch_bounds__(NPTL_STAGED_BLOCK) void k_step_nptl_staged(StepParams p)
+constexpr uint32_t nptl_staged_rows()
+{
+    if constexpr (L != LBM_LAYOUT_COLLISION)
+        return 1u;
+    else if constexpr (S == LBM_STORE_F64)
+        return 8u;
+    else if constexpr (S == LBM_STORE_F32)
+        return 4u;
+    else
+        return 2u;
+}
+
+template <int L, int S>
+__global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) void k_step_nptl_staged(StepParams p)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
     using V4 = unsigned int __attribute__((ext_vector_type(4)));
     constexpr uint32_t B = NPTL_STAGED_BLOCK;
+    constexpr uint32_t R = nptl_staged_rows<L, S>();               // node rows per block
+    constexpr uint32_t NT = B * R;                                 // threads per block
     constexpr uint32_t VE = 16u / sizeof(T);                       // elements per 16-byte vector
     constexpr uint32_t K = L == LBM_LAYOUT_COLLISION ? 9u : 3u;    // staged populations per place
+    constexpr uint32_t NR = R + 2u;                                // staged ranges
     constexpr uint32_t ROW = (K * (B + 2u) + 2u * VE + VE - 1u) / VE * VE;
-    __shared__ __align__(16) T stage[3 * ROW];
+    __shared__ __align__(16) T stage[NR * ROW];
     const T *src = static_cast<const T *>(p.src);
     T *dst = static_cast<T *>(p.dst);
-    const uint32_t tid = threadIdx.x;
-    const uint32_t x0 = 1u + blockIdx.x * B, y = 1u + blockIdx.y;
+    const uint32_t tid = threadIdx.x, tx = tid % B, ty = tid / B;
+    const uint32_t x0 = 1u + blockIdx.x * B, y0 = 1u + blockIdx.y * R;
     const uint32_t len = min(B, p.Hx - 1u - x0);
-    const uint32_t b0 = place(x0, y, p.P, p.X0);
-    uint32_t shift[3];
+    const uint32_t rows = min(R, p.Vx - 1u - y0);                  // node rows of this block
+    const uint32_t b0 = place(x0, y0, p.P, p.X0);
+    uint32_t shift[NR];
 #pragma unroll
-    for (uint32_t r = 0; r < 3; ++r)
+    for (uint32_t r = 0; r < NR; ++r)
     {
+        // collision: range r holds row y0 - 1 + r; bundle (R = 1): range r holds
+        // array r of row y0 + 1 - r
         uint32_t e0;
         if constexpr (L == LBM_LAYOUT_COLLISION)
             e0 = 9u * (b0 + r * p.P - p.P - 1u);
@@ -211,15 +236,20 @@ __global__ __launch_bounds__(NPTL_STAGED_BLOCK) void k_step_nptl_staged(StepPara
             e0 = 3u * r * p.S + 3u * (b0 + p.P - r * p.P - 1u);
         const uint32_t ea = e0 - e0 % VE;
         shift[r] = e0 - ea;
-        const uint32_t nvec = (shift[r] + K * (len + 2u) + VE - 1u) / VE;
-        const V4 *g = reinterpret_cast<const V4 *>(src + ea);
-        V4 *l = reinterpret_cast<V4 *>(stage + r * ROW);
-        for (uint32_t i = tid; i < nvec; i += B)
-            l[i] = g[i];
+        if (r < rows + 2u)
+        {
+            const uint32_t nvec = (shift[r] + K * (len + 2u) + VE - 1u) / VE;
+            const V4 *g = reinterpret_cast<const V4 *>(src + ea);
+            V4 *l = reinterpret_cast<V4 *>(stage + r * ROW);
+            for (uint32_t i = tid; i < nvec; i += NT)
+                l[i] = g[i];
+        }
     }
     __syncthreads();
-    const uint32_t pn = b0 + tid;
-    const uint32_t code = tid < len ? p.code[pn] : 0u;
+    const uint32_t y = y0 + ty;
+    const uint32_t pn = b0 + ty * p.P + tx;
+    const bool active = tx < len && ty < rows;
+    const uint32_t code = active ? p.code[pn] : 0u;
     const bool fluid = code & LBM_CODE_FLUID;
     const bool solid = code & LBM_CODE_SOLID;
     C f[9] = {};
@@ -231,8 +261,8 @@ __global__ __launch_bounds__(NPTL_STAGED_BLOCK) void k_step_nptl_staged(StepPara
             const bool bb = (code >> d) & 1u;
             if constexpr (L == LBM_LAYOUT_COLLISION)
             {
-                const uint32_t r = bb ? 1u : uint32_t(1 - cy_of(d));
-                const uint32_t i = bb ? tid + 1u : uint32_t(int32_t(tid) + 1 - cx_of(d));
+                const uint32_t r = bb ? ty + 1u : uint32_t(int32_t(ty) + 1 - cy_of(d));
+                const uint32_t i = bb ? tx + 1u : uint32_t(int32_t(tx) + 1 - cx_of(d));
                 const int dd = bb ? 8 - d : d;
                 f[d] = Storage<S>::load(stage + r * ROW + shift[r], 9u * i + uint32_t(dd), dd);
             }
@@ -241,35 +271,36 @@ __global__ __launch_bounds__(NPTL_STAGED_BLOCK) void k_step_nptl_staged(StepPara
                 if (!bb)
                 {
                     const uint32_t r = uint32_t(d / 3);
-                    const uint32_t i = uint32_t(int32_t(tid) + 1 - cx_of(d));
+                    const uint32_t i = uint32_t(int32_t(tx) + 1 - cx_of(d));
                     f[d] = Storage<S>::load(stage + r * ROW + shift[r], 3u * i + uint32_t(d % 3), d);
                 }
                 else if ((8 - d) / 3 == 1)
-                    f[d] = Storage<S>::load(stage + ROW + shift[1], 3u * (tid + 1u) + uint32_t((8 - d) % 3), 8 - d);
+                    f[d] = Storage<S>::load(stage + ROW + shift[1], 3u * (tx + 1u) + uint32_t((8 - d) % 3), 8 - d);
                 else
                     f[d] = Storage<S>::load(src, at<L>(pn, uint32_t(8 - d), p.S), 8 - d);
             }
         }
-        finish_node<L, S>(p, code, x0 + tid, y, f);
+        finish_node<L, S>(p, code, x0 + tx, y, f);
     }
-    const bool odd = tid < len && !fluid && !solid;
+    const bool odd = active && !fluid && !solid;
     if (__syncthreads_or(odd))
     {
         if (fluid || solid)
             store_node<L, S>(dst, pn, p.S, f, p.sr);
         return;
     }
-    // results through LDS (stage range 0 is free after the barrier above)
+    // results through LDS (the staged ranges are free after the barrier above):
+    // collision: node row ty at ty * 9B; bundle: array a at a * 3B
     T *out = stage;
-    if (tid < len)
+    if (active)
     {
         const uint32_t h = p.sr ? mix32(p.sr ^ pn) : 0u;
         uint32_t hh = h;
 #pragma unroll
         for (int d = 0; d < 9; ++d)
         {
-            const uint32_t o = L == LBM_LAYOUT_COLLISION ? 9u * tid + uint32_t(d)
-                                                         : uint32_t(d / 3) * (3u * B) + 3u * tid + uint32_t(d % 3);
+            const uint32_t o = L == LBM_LAYOUT_COLLISION ? ty * (9u * B) + 9u * tx + uint32_t(d)
+                                                         : uint32_t(d / 3) * (3u * B) + 3u * tx + uint32_t(d % 3);
             if constexpr (S == LBM_STORE_F16_SHIFTED || S == LBM_STORE_BF16_SHIFTED)
             {
                 if (p.sr)
@@ -283,19 +314,19 @@ __global__ __launch_bounds__(NPTL_STAGED_BLOCK) void k_step_nptl_staged(StepPara
         }
     }
     __syncthreads();
-    constexpr uint32_t ARR = L == LBM_LAYOUT_COLLISION ? 1u : 3u;  // contiguous output arrays
+    // contiguous output arrays: one per node row (collision), three (bundle)
+    const uint32_t arrays = L == LBM_LAYOUT_COLLISION ? rows : 3u;
     const uint32_t per = K * len;                                  // elements per array
-#pragma unroll
-    for (uint32_t a = 0; a < ARR; ++a)
+    for (uint32_t a = 0; a < arrays; ++a)
     {
-        const uint32_t g0 = L == LBM_LAYOUT_COLLISION ? 9u * b0 : 3u * a * p.S + 3u * b0;
-        const T *o = out + a * (3u * B);
+        const uint32_t g0 = L == LBM_LAYOUT_COLLISION ? 9u * (b0 + a * p.P) : 3u * a * p.S + 3u * b0;
+        const T *o = out + a * (K * B);
         const uint32_t nvec = per / VE;
         const V4 *lo = reinterpret_cast<const V4 *>(o);
         V4 *go = reinterpret_cast<V4 *>(dst + g0);
-        for (uint32_t i = tid; i < nvec; i += B)
+        for (uint32_t i = tid; i < nvec; i += NT)
             __builtin_nontemporal_store(lo[i], go + i);
-        for (uint32_t i = nvec * VE + tid; i < per; i += B)
+        for (uint32_t i = nvec * VE + tid; i < per; i += NT)
             dst[g0 + i] = o[i];
     }
 }
@@ -738,8 +769,9 @@ struct Impl
         {
             if constexpr (nptl_staged<L, S>())
             {
-                dim3 grid((c->Hx - 2u + NPTL_STAGED_BLOCK - 1u) / NPTL_STAGED_BLOCK, c->Vx - 2u);
-                hipLaunchKernelGGL((k_step_nptl_staged<L, S>), grid, dim3(NPTL_STAGED_BLOCK), 0, c->stream, p);
+                constexpr uint32_t rows = nptl_staged_rows<L, S>();
+                dim3 grid((c->Hx - 2u + NPTL_STAGED_BLOCK - 1u) / NPTL_STAGED_BLOCK, (c->Vx - 2u + rows - 1u) / rows);
+                hipLaunchKernelGGL((k_step_nptl_staged<L, S>), grid, dim3(NPTL_STAGED_BLOCK * rows), 0, c->stream, p);
                 break;
             }
             constexpr uint32_t R = nptl_rows<S>();
