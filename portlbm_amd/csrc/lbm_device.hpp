@@ -264,7 +264,12 @@ __device__ __forceinline__ void zou_he_outlet(C f[9], C rho_out, C uy_prev)
 // Moments (rho = sum f, j = sum f c) and BGK relaxation towards the
 // incompressible equilibrium feq = w (rho + 3 c.j + 4.5 (c.j)^2 - 1.5 |j|^2);
 // f' = -(1/tau)(f - feq) + f.  Velocity is the momentum j (never divided by
-// rho).  Term order as in nptl_kernels.hpp:382-405.
+// rho).  Term order as in nptl_kernels.hpp:382-405, with two rewrites that
+// leave every finite result bit for bit unchanged (at most the sign of an
+// exact zero in j differs): terms with a zero velocity component are left
+// out (x + 0 * f == x; without fast-math the compiler may not drop them), and
+// a direction and its opposite share 3 c.j and 4.5 (c.j)^2, since c.j of the
+// opposite is the exact negation and (4.5 (-a)) (-a) == (4.5 a) a.
 template <class C>
 __device__ __forceinline__ void moments(const C f[9], C &rho, C &jx, C &jy, C &usq)
 {
@@ -275,23 +280,45 @@ __device__ __forceinline__ void moments(const C f[9], C &rho, C &jx, C &jy, C &u
     for (int d = 0; d < 9; ++d)
     {
         rho += f[d];
-        jx += f[d] * C(cx_of(d));
-        jy += f[d] * C(cy_of(d));
+        if (cx_of(d) != 0)
+            jx += f[d] * C(cx_of(d));
+        if (cy_of(d) != 0)
+            jy += f[d] * C(cy_of(d));
     }
     usq = jx * jx + jy * jy;
+}
+
+// c_d . j with the reference's term order, zero components left out
+template <class C>
+__device__ __forceinline__ C c_dot(int d, C jx, C jy)
+{
+    if (cx_of(d) != 0 && cy_of(d) != 0)
+        return C(cx_of(d)) * jx + C(cy_of(d)) * jy;
+    else if (cx_of(d) != 0)
+        return C(cx_of(d)) * jx;
+    else
+        return C(cy_of(d)) * jy;
 }
 
 template <class C>
 __device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C &usq)
 {
     moments<C>(f, rho, jx, jy, usq);
+    const C s = C(1.5) * usq;
+    C feq[9];
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+    {
+        const C cu = c_dot<C>(d, jx, jy);
+        const C t = C(3) * cu;
+        const C q = C(4.5) * cu * cu;
+        feq[d] = weight<C>(d) * (rho + t + q - s);
+        feq[8 - d] = weight<C>(8 - d) * (rho - t + q - s);
+    }
+    feq[4] = weight<C>(4) * (rho - s);
 #pragma unroll
     for (int d = 0; d < 9; ++d)
-    {
-        const C cu = C(cx_of(d)) * jx + C(cy_of(d)) * jy;
-        const C feq = weight<C>(d) * (rho + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
-        f[d] = -omega * (f[d] - feq) + f[d];
-    }
+        f[d] = -omega * (f[d] - feq[d]) + f[d];
 }
 
 // In the stream layout every store instruction of a wave writes whole

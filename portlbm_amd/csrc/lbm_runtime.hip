@@ -171,11 +171,15 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
 // is neither fluid nor solid (the walls stencil's value 2 on tiny grids)
 // writes node by node instead and leaves that node untouched.
 // Used where it measured faster than the plain kernel on the 8192 x 2048
-// porous channel (64-node segments; GFLUPS plain -> staged): FP64 collision
-// 16.3 -> 21.2 (R = 8; R = 2, 4: 18.3, 20.3), FP32 collision 25.5 -> 45.7
-// (R = 4; R = 1, 2, 8: 34.0, 41.0, 34.6), FP16 collision 43.6 -> 64.9 (R = 2;
-// R = 1, 4, 8: 62.7, 53.3, 38.7), FP64 bundle 24.2 -> 28.1, FP32 bundle
-// 42.8 -> 48.8.  The 16-bit bundle layout keeps the plain kernel.
+// porous channel (64-node segments; GFLUPS plain -> staged, range-by-range
+// copy into LDS): FP64 collision 16.3 -> 21.2 (R = 8; R = 2, 4: 18.3, 20.3),
+// FP32 collision 25.5 -> 45.7 (R = 4; R = 1, 2, 8: 34.0, 41.0, 34.6), FP16
+// collision 43.6 -> 64.9 (R = 2; R = 1, 4, 8: 62.7, 53.3, 38.7), FP64 bundle
+// 24.2 -> 28.1, FP32 bundle 42.8 -> 48.8; the flat copy below then takes FP32
+// and FP16 collision to 50.2 and 67.4.  The 16-bit bundle and stream layouts
+// keep the plain kernel (staging the stream layout measured 58.5 against
+// 88.8 GFLUPS in FP16: the bounce-back pulls of the six directions with
+// cy != 0 read rows that are not staged).
 constexpr uint32_t NPTL_STAGED_BLOCK = 64;
 
 template <int L, int S>
@@ -187,6 +191,18 @@ constexpr bool nptl_staged()
         return sizeof(typename Storage<S>::T) >= 4;
     else
         return false;
+}
+
+// How the source ranges reach LDS: "flat" makes every 16-byte vector of every
+// range one work item and issues all loads of a thread before its LDS writes
+// (one wait for HBM per block); otherwise the block copies range by range.
+// Flat measured faster for the collision layout in FP32 (45.5 -> 49.9 GFLUPS)
+// and 16-bit (65.0 -> 66.9), slower for FP64 collision (21.2 -> 11.8) and
+// FP64 bundle (28.1 -> 20.3).
+template <int L, int S>
+constexpr bool nptl_staged_flat()
+{
+    return L == LBM_LAYOUT_COLLISION && sizeof(typename Storage<S>::T) <= 4;
 }
 
 template <int L, int S>
@@ -213,7 +229,7 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
     constexpr uint32_t NT = B * R;                                 // threads per block
     constexpr uint32_t VE = 16u / sizeof(T);                       // elements per 16-byte vector
     constexpr uint32_t K = L == LBM_LAYOUT_COLLISION ? 9u : 3u;    // staged populations per place
-    constexpr uint32_t NR = R + 2u;                                // staged ranges
+    constexpr uint32_t NR = L == LBM_LAYOUT_COLLISION ? R + 2u : 3u;  // staged ranges
     constexpr uint32_t ROW = (K * (B + 2u) + 2u * VE + VE - 1u) / VE * VE;
     __shared__ __align__(16) T stage[NR * ROW];
     const T *src = static_cast<const T *>(p.src);
@@ -223,26 +239,66 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
     const uint32_t len = min(B, p.Hx - 1u - x0);
     const uint32_t rows = min(R, p.Vx - 1u - y0);                  // node rows of this block
     const uint32_t b0 = place(x0, y0, p.P, p.X0);
+    // first element of range r: collision: row y0 - 1 + r; bundle (R = 1):
+    // array r of row y0 + 1 - r
+    auto range_start = [&](uint32_t r) -> uint32_t {
+        if constexpr (L == LBM_LAYOUT_COLLISION)
+            return 9u * (b0 + r * p.P - p.P - 1u);
+        else
+            return 3u * r * p.S + 3u * (b0 + p.P - r * p.P - 1u);
+    };
+    const uint32_t staged = L == LBM_LAYOUT_COLLISION ? rows + 2u : NR;
     uint32_t shift[NR];
 #pragma unroll
     for (uint32_t r = 0; r < NR; ++r)
+        shift[r] = range_start(r) % VE;
+    if constexpr (nptl_staged_flat<L, S>())
     {
-        // collision: range r holds row y0 - 1 + r; bundle (R = 1): range r holds
-        // array r of row y0 + 1 - r
-        uint32_t e0;
-        if constexpr (L == LBM_LAYOUT_COLLISION)
-            e0 = 9u * (b0 + r * p.P - p.P - 1u);
-        else
-            e0 = 3u * r * p.S + 3u * (b0 + p.P - r * p.P - 1u);
-        const uint32_t ea = e0 - e0 % VE;
-        shift[r] = e0 - ea;
-        if (r < rows + 2u)
+        // Every vector of every range is one work item, at most NVR per range;
+        // a thread issues all of its loads before it writes any to LDS, so
+        // the block waits for HBM once.
+        constexpr uint32_t NVR = (K * (B + 2u) + 2u * (VE - 1u)) / VE;
+        constexpr uint32_t PT = (NR * NVR + NT - 1u) / NT;
+        const uint32_t nvec_max = (K * (len + 2u) + 2u * (VE - 1u)) / VE;
+        V4 tmp[PT];
+        uint32_t slot[PT];
+#pragma unroll
+        for (uint32_t j = 0; j < PT; ++j)
         {
-            const uint32_t nvec = (shift[r] + K * (len + 2u) + VE - 1u) / VE;
-            const V4 *g = reinterpret_cast<const V4 *>(src + ea);
-            V4 *l = reinterpret_cast<V4 *>(stage + r * ROW);
-            for (uint32_t i = tid; i < nvec; i += NT)
-                l[i] = g[i];
+            const uint32_t v = tid + j * NT;
+            const uint32_t r = v / NVR, i = v - r * NVR;
+            slot[j] = ~0u;
+            if (r < staged && i < nvec_max)
+            {
+                const uint32_t e0 = range_start(r);
+                const uint32_t ea = e0 - e0 % VE;
+                if (i < (e0 - ea + K * (len + 2u) + VE - 1u) / VE)
+                {
+                    tmp[j] = reinterpret_cast<const V4 *>(src + ea)[i];
+                    slot[j] = r * (ROW / VE) + i;
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < PT; ++j)
+            if (slot[j] != ~0u)
+                reinterpret_cast<V4 *>(stage)[slot[j]] = tmp[j];
+    }
+    else
+    {
+        // range by range, all threads on one range at a time
+#pragma unroll
+        for (uint32_t r = 0; r < NR; ++r)
+        {
+            if (r < staged)
+            {
+                const uint32_t ea = range_start(r) - shift[r];
+                const uint32_t nvec = (shift[r] + K * (len + 2u) + VE - 1u) / VE;
+                const V4 *g = reinterpret_cast<const V4 *>(src + ea);
+                V4 *l = reinterpret_cast<V4 *>(stage + r * ROW);
+                for (uint32_t i = tid; i < nvec; i += NT)
+                    l[i] = g[i];
+            }
         }
     }
     __syncthreads();
