@@ -819,15 +819,21 @@ struct Impl
     static int step(lbm_ctx *c, bool macros)
     {
         const StepParams p = params(c, macros);
+        auto launch_staged = [&]() {
+            if constexpr (nptl_staged<L, S>())
+            {
+                constexpr uint32_t rows = nptl_staged_rows<L, S>();
+                dim3 grid((c->Hx - 2u + NPTL_STAGED_BLOCK - 1u) / NPTL_STAGED_BLOCK, (c->Vx - 2u + rows - 1u) / rows);
+                hipLaunchKernelGGL((k_step_nptl_staged<L, S>), grid, dim3(NPTL_STAGED_BLOCK * rows), 0, c->stream, p);
+            }
+        };
         switch (c->desc.algorithm)
         {
         case LBM_ALG_NPTL:
         {
             if constexpr (nptl_staged<L, S>())
             {
-                constexpr uint32_t rows = nptl_staged_rows<L, S>();
-                dim3 grid((c->Hx - 2u + NPTL_STAGED_BLOCK - 1u) / NPTL_STAGED_BLOCK, (c->Vx - 2u + rows - 1u) / rows);
-                hipLaunchKernelGGL((k_step_nptl_staged<L, S>), grid, dim3(NPTL_STAGED_BLOCK * rows), 0, c->stream, p);
+                launch_staged();
                 break;
             }
             constexpr uint32_t R = nptl_rows<S>();
@@ -837,6 +843,14 @@ struct Impl
         }
         case LBM_ALG_LPTL:
         {
+            // Same two-lattice update on the same placement; where the staged
+            // kernel is faster (collision, FP64/FP32 bundle) lptl runs it too
+            // and the work-group size of the settings does not apply.
+            if constexpr (nptl_staged<L, S>())
+            {
+                launch_staged();
+                break;
+            }
             const uint32_t b = c->wg;
             const uint32_t range = c->P * c->Vx;
             hipLaunchKernelGGL((k_step_lptl<L, S>), dim3((range + b - 1u) / b), dim3(b), 0, c->stream, p);
