@@ -41,6 +41,28 @@ def test_library_exports_every_declared_symbol(header, lib):
     assert not missing, missing
 
 
+@pytest.mark.parametrize("change,message", [
+    ({"horizontal_nodes": 3}, "must be >= 4"),
+    ({"relaxation_time": 0.0}, "relaxation_time must be > 0"),
+    ({"work_group_size": 2048}, r"work_group_size must be in \[1, 1024\]"),
+    ({"storage": 7}, "unknown storage type"),
+    ({"rounding": 2}, "unknown rounding mode"),
+    ({"expanded_horizontal_nodes": 10}, "expanded extents smaller than the lattice"),
+    # 9 x 40000 x 20000 populations do not fit 32-bit indices
+    ({"horizontal_nodes": 40000, "vertical_nodes": 20000, "expanded_horizontal_nodes": 40000,
+      "expanded_vertical_nodes": 20000}, "too large for 32-bit population indices"),
+])
+def test_lbm_create_rejects_bad_descriptors_before_any_device_call(change, message):
+    """lbm_create validates the descriptor first (no HIP call), so the checks
+    run here without a GPU; the error surfaces as LbmError with the message."""
+    desc = pa.desc_from_settings(pa.settings("nptl", "stream", inner_h=64, inner_v=32))
+    for key, value in change.items():
+        setattr(desc, key, value)
+    with pytest.raises(pa.LbmError, match=message) as err:
+        pa.Lattice(desc)
+    assert err.value.code == 1  # LBM_ERR_INVALID_ARGUMENT
+
+
 def test_bundled_settings_file():
     doc = json.loads((ROOT / "settings" / "settings.json").read_text())
     pa.validate(doc)
