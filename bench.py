@@ -108,7 +108,17 @@ def measured_traffic(algorithm: str, layout: str, storage: str) -> float | None:
     return None if rec is None else float(rec["bytes_per_launch"])
 
 
-def sweep(out_path: str, steps: int, warmup: int) -> None:
+def sweep(out_path: str, steps: int, warmup: int, cpu_budget_s: float = 0.0) -> None:
+    """Every algorithm x layout x storage on the default workload.  With
+    cpu_budget_s > 0 the CPU oracle (FP64, the reference's pass structure) is
+    timed beside each algorithm x layout for about that many seconds."""
+    cpu = {}
+    if cpu_budget_s > 0:
+        for algorithm in ("nptl", "lptl", "npol", "nsol"):
+            for layout in ("collision", "stream", "bundle"):
+                cpu[(algorithm, layout)] = cpu_baseline(algorithm, layout, cpu_budget_s)
+                print(json.dumps({"cpu": [algorithm, layout], **cpu[(algorithm, layout)]}), file=sys.stderr,
+                      flush=True)
     rows = []
     for storage in ("f64", "f32", "f16-shifted", "bf16-shifted"):
         for algorithm in ("nptl", "lptl", "npol", "nsol"):
@@ -120,6 +130,10 @@ def sweep(out_path: str, steps: int, warmup: int) -> None:
                        "mlups_all": round(INNER_H * INNER_V * steps / (r["dev_ms"] * 1e-3) / 1e6, 1),
                        "gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4),
                        "ms_per_step": round(r["dev_ms"] / steps, 4)}
+                if (algorithm, layout) in cpu:
+                    c = cpu[(algorithm, layout)]
+                    row["cpu_mflups_fp64"] = c["value"]
+                    row["cpu_cores"] = c["cores"]
                 rows.append(row)
                 print(json.dumps(row), file=sys.stderr, flush=True)
     Path(out_path).write_text(json.dumps({"workload": f"porous-circles {INNER_H}x{INNER_V}", "steps": steps,
@@ -282,6 +296,8 @@ def main() -> int:
                     help="porous-circles (the benchmark workload) or any other scenario, for comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep", metavar="OUT.json")
+    ap.add_argument("--sweep-cpu-seconds", type=float, default=0.0,
+                    help="time the CPU oracle beside each algorithm x layout of the sweep for about this long")
     ap.add_argument("--precision-study", metavar="OUT.json")
     ap.add_argument("--study-scenario", default="cylinder", choices=("cylinder", "porous"))
     ap.add_argument("--study-taus", default="0.52,0.6,0.8")
@@ -298,7 +314,7 @@ def main() -> int:
         return 2
 
     if args.sweep:
-        sweep(args.sweep, min(args.steps, 500), min(args.warmup, 20))
+        sweep(args.sweep, min(args.steps, 500), min(args.warmup, 20), args.sweep_cpu_seconds)
     if args.precision_study:
         taus = tuple(float(t) for t in args.study_taus.split(","))
         precision_study(args.precision_study, args.study_steps, taus, args.study_scenario)
