@@ -12,8 +12,17 @@ in HBM.  The unit of work is a fluid-node update; the line reports MFLUPS
 against the MI355X peak (algorithmic bytes: 2 x 9 x sizeof(storage) per
 fluid node), and the CPU oracle timed on the host cores as a baseline.
 
-Prints one JSON line.  --sweep additionally times every algorithm x layout x
-storage combination and writes them to a JSON file.
+Prints one JSON line.  Optional studies, each written to its own JSON file
+before the line is printed:
+  --sweep OUT            every algorithm x layout x storage combination
+                         (--sweep-cpu-seconds S: the CPU oracle beside each
+                         algorithm x layout for about S seconds)
+  --precision-study OUT  FP32/FP16/BF16 storage against FP64 after a fixed
+                         number of steps (--study-scenario, --study-taus,
+                         --study-steps)
+  --steady-study OUT     the same at the steady state of a 1024x256 channel
+  --batched OUT          512 uneven channels: per-domain launches, one
+                         persistent launch per step, autotuned graph replay
 """
 from __future__ import annotations
 
