@@ -25,6 +25,8 @@
  *      per step plus, for the one-lattice schemes, one halo refresh)
  *   SYCLAlgorithm::copy_macroscopic_observables_to_cpu   lbm_get_macros
  *     include/lbm/execution/sycl_algorithm.hpp:90-104
+ *   the same copies enqueued after each frame step       lbm_run_frames /
+ *     include/lbm/gpu/nptl/nptl.hpp:71-75                  lbm_get_frame
  *   Data::distribution_values_0 (device lattice)         lbm_get_populations /
  *     include/lbm/core/simulation.hpp:240-254              lbm_set_populations
  *   Data::phase_information                              lbm_get_phase
@@ -191,6 +193,22 @@ int lbm_last_run_ms(lbm_ctx *ctx, double *ms);
  * be NULL to skip that array. */
 int lbm_get_macros(lbm_ctx *ctx, double *density, double *x_velocity, double *y_velocity,
                    double *absolute_velocity, size_t count);
+
+/* lbm_run with asynchronous frame copies, the reference's non-blocking
+ * queue->copy of the results after each frame step (nptl.hpp:71-75,
+ * sycl_algorithm.hpp:90-104): after every frame step the four result arrays
+ * are copied to pinned host memory on a second stream while the steps go on;
+ * the next step that writes results waits for that copy only.  The call
+ * returns when the steps and the copies are done; lbm_get_frame then hands out
+ * the latest frame.  Same populations and results as lbm_run. */
+int lbm_run_frames(lbm_ctx *ctx, uint32_t steps, uint32_t frame_interval);
+
+/* The result arrays of the latest frame copied by lbm_run_frames (layout as
+ * lbm_get_macros; NULL pointers skip an array) and its iteration (the frame
+ * step's global iteration, counted from 0).  LBM_ERR_STATE before the first
+ * frame. */
+int lbm_get_frame(lbm_ctx *ctx, double *density, double *x_velocity, double *y_velocity,
+                  double *absolute_velocity, size_t count, uint64_t *iteration);
 
 /* Current source lattice: 9 * expanded node count values in the context's
  * layout order (core/access.hpp:117-171), widened to double; shifted

@@ -112,6 +112,8 @@ def hip_library() -> ctypes.CDLL:
             "lbm_get_iteration": ([_P, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "lbm_last_run_ms": ([_P, _DP], ctypes.c_int),
             "lbm_get_macros": ([_P, _P, _P, _P, _P, _SZ], ctypes.c_int),
+            "lbm_run_frames": ([_P, _U32, _U32], ctypes.c_int),
+            "lbm_get_frame": ([_P, _P, _P, _P, _P, _SZ, _P], ctypes.c_int),
             "lbm_get_populations": ([_P, _P, _SZ], ctypes.c_int),
             "lbm_set_populations": ([_P, _P, _SZ], ctypes.c_int),
             "lbm_set_equilibrium_field": ([_P, _P, _P, _P, _SZ], ctypes.c_int),
@@ -369,6 +371,21 @@ class Lattice:
         out = [np.empty(self.D) for _ in range(4)]
         _check_hip(self.lib.lbm_get_macros(self._ctx, *map(_ptr, out), self.D), "lbm_get_macros")
         return tuple(out)  # type: ignore[return-value]
+
+    def run_frames(self, steps: int, frame_interval: int) -> float:
+        """lbm_run with the frame copies made on a second stream while the
+        steps go on; returns the device milliseconds of the call."""
+        _check_hip(self.lib.lbm_run_frames(self._ctx, steps, frame_interval), "lbm_run_frames")
+        ms = ctypes.c_double(0)
+        _check_hip(self.lib.lbm_last_run_ms(self._ctx, ctypes.byref(ms)), "lbm_last_run_ms")
+        return ms.value
+
+    def frame(self) -> tuple[int, tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]]:
+        """(iteration, (rho, jx, jy, |j|)) of the latest frame copied by run_frames."""
+        out = [np.empty(self.D) for _ in range(4)]
+        it = ctypes.c_uint64(0)
+        _check_hip(self.lib.lbm_get_frame(self._ctx, *map(_ptr, out), self.D, ctypes.byref(it)), "lbm_get_frame")
+        return it.value, tuple(out)  # type: ignore[return-value]
 
     def populations(self) -> np.ndarray:
         out = np.empty(9 * self.N)

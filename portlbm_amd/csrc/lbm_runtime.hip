@@ -71,6 +71,12 @@ struct lbm_ctx
     double *red_out = nullptr;
     uint32_t D;               // (H-2)*(V-2)
     hipStream_t stream = nullptr;
+    // asynchronous frame copies (lbm_run_frames; created on first use)
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_frame = nullptr, ev_copied = nullptr;
+    double *frame_host = nullptr;  // pinned, 4 * D
+    bool copy_pending = false;     // a copy of the results may still be reading them
+    uint64_t frame_iteration = ~0ull;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t iteration = 0;
     bool phase_set = false;
@@ -1128,6 +1134,17 @@ void lbm_destroy(lbm_ctx *c)
         (void)hipEventDestroy(c->ev0);
     if (c->ev1)
         (void)hipEventDestroy(c->ev1);
+    if (c->copy_stream)
+    {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    if (c->ev_frame)
+        (void)hipEventDestroy(c->ev_frame);
+    if (c->ev_copied)
+        (void)hipEventDestroy(c->ev_copied);
+    if (c->frame_host)
+        (void)hipHostFree(c->frame_host);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1267,6 +1284,7 @@ int lbm_init_populations(lbm_ctx *c)
     LBM_HIP_TRY(hipStreamSynchronize(c->stream));
     c->iteration = 0;
     c->snap_valid = false;
+    c->frame_iteration = ~0ull;
     c->initialized = true;
     return LBM_OK;
 }
@@ -1315,6 +1333,76 @@ int lbm_run(lbm_ctx *c, uint32_t steps, uint32_t frame_interval)
     float ms = 0.f;
     LBM_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    return LBM_OK;
+}
+
+int lbm_run_frames(lbm_ctx *c, uint32_t steps, uint32_t frame_interval)
+{
+    if (!c)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_run_frames: null context");
+    if (!c->initialized)
+        return fail(LBM_ERR_STATE, "lbm_run_frames: call lbm_init_populations first");
+    if (!c->copy_stream)
+    {
+        LBM_HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        LBM_HIP_TRY(hipEventCreateWithFlags(&c->ev_frame, hipEventDisableTiming));
+        LBM_HIP_TRY(hipEventCreateWithFlags(&c->ev_copied, hipEventDisableTiming));
+        LBM_HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&c->frame_host),
+                                  4ull * std::max(1u, c->D) * sizeof(double), hipHostMallocDefault));
+    }
+    const VTable vt = vtable(c);
+    uint64_t last_frame = ~0ull;
+    LBM_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (uint32_t i = 0; i < steps; ++i)
+    {
+        const uint64_t it = c->iteration;
+        const bool frame = frame_interval ? (it % frame_interval == frame_interval - 1u) : false;
+        const bool macros = frame || i + 1u == steps;
+        if (macros && c->copy_pending)
+        {
+            // the results must not change under the copy still reading them
+            LBM_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_copied, 0));
+            c->copy_pending = false;
+        }
+        if (int rc = vt.step(c, macros))
+            return rc;
+        if (frame)
+        {
+            LBM_HIP_TRY(hipEventRecord(c->ev_frame, c->stream));
+            LBM_HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->ev_frame, 0));
+            LBM_HIP_TRY(hipMemcpyAsync(c->frame_host, c->res, 4ull * c->D * sizeof(double), hipMemcpyDeviceToHost,
+                                       c->copy_stream));
+            LBM_HIP_TRY(hipEventRecord(c->ev_copied, c->copy_stream));
+            c->copy_pending = true;
+            last_frame = it;
+        }
+    }
+    LBM_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    LBM_HIP_TRY(hipEventSynchronize(c->ev1));
+    LBM_HIP_TRY(hipStreamSynchronize(c->copy_stream));
+    c->copy_pending = false;
+    if (last_frame != ~0ull)
+        c->frame_iteration = last_frame;
+    float ms = 0.f;
+    LBM_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return LBM_OK;
+}
+
+int lbm_get_frame(lbm_ctx *c, double *rho, double *jx, double *jy, double *jabs, size_t count, uint64_t *iteration)
+{
+    if (!c)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_frame: null context");
+    if (count != c->D)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_frame: count must equal (H-2)*(V-2)");
+    if (c->frame_iteration == ~0ull)
+        return fail(LBM_ERR_STATE, "lbm_get_frame: no frame has been copied (lbm_run_frames)");
+    double *dst[4] = {rho, jx, jy, jabs};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k] && count)
+            std::memcpy(dst[k], c->frame_host + size_t(k) * c->D, count * sizeof(double));
+    if (iteration)
+        *iteration = c->frame_iteration;
     return LBM_OK;
 }
 
@@ -1464,6 +1552,7 @@ int lbm_set_state(lbm_ctx *c, const void *buf, size_t bytes)
     LBM_HIP_TRY(hipMemcpyAsync(c->res, in + off, 4ull * c->D * sizeof(double), hipMemcpyHostToDevice, c->stream));
     LBM_HIP_TRY(hipStreamSynchronize(c->stream));
     c->iteration = iteration;
+    c->frame_iteration = ~0ull;
     c->cur = int(cur);
     c->snap_valid = false;
     return LBM_OK;

@@ -186,8 +186,11 @@ void HipAlgorithmHandler::copy_macroscopic_observables_to_cpu()
           "lbm_get_macros");
 }
 
-// The reference's execute() loop (nptl.hpp:116-141) in batches: each batch
-// ends at the next frame step, the last step, or after max_batch steps.
+// The reference's execute() loop (nptl.hpp:116-141) in batches of at most
+// max_batch steps.  Frame steps inside a batch copy the results to pinned
+// host memory on a second stream while the steps go on, as the reference's
+// non-blocking queue->copy does (nptl.hpp:71-75); after the batch the latest
+// of those frames goes to the host vectors.
 void HipAlgorithmHandler::execute()
 {
     const unsigned int fui = properties_->frame_update_interval;
@@ -195,17 +198,21 @@ void HipAlgorithmHandler::execute()
     {
         const unsigned int it = control_->get_current_iteration();
         const unsigned int remaining = control_->get_max_iterations() - it;
-        const unsigned int to_frame = fui ? fui - (it % fui) : remaining;
-        const unsigned int batch = std::min({remaining, to_frame, max_batch});
+        const unsigned int batch = std::min(remaining, max_batch);
         control_->reset_timer();
-        check(lbm_run(ctx_, batch, fui), "lbm_run");
+        check(lbm_run_frames(ctx_, batch, fui), "lbm_run_frames");
         double ms = 0.0;
         check(lbm_last_run_ms(ctx_, &ms), "lbm_last_run_ms");
         device_ms_ += ms;
         control_->finalize_iterations(batch, control_->elapsed_ms());
-        const unsigned int last = it + batch - 1;
-        if (fui && last % fui == fui - 1)
-            copy_macroscopic_observables_to_cpu();
+        // first frame step at or after it: it + (fui - 1 - it % fui)
+        if (fui && (fui - 1 - it % fui) < batch)
+        {
+            std::uint64_t frame = 0;
+            check(lbm_get_frame(ctx_, densities_.data(), x_velocities_.data(), y_velocities_.data(),
+                                absolute_velocities_.data(), densities_.size(), &frame),
+                  "lbm_get_frame");
+        }
     }
     if (control_->is_finished())
         copy_macroscopic_observables_to_cpu();
