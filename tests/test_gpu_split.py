@@ -46,6 +46,25 @@ def test_split_steps_equal_fused_steps(alg, layout, storage, rounding):
         np.testing.assert_array_equal(split.populations()[keep], fused.populations()[keep])
 
 
+# The staged kernels (LDS row staging, flat or range-by-range copy, 8/4/2 node
+# rows per block) on shapes that leave partial row blocks and partial 64-node
+# segments, against the split passes, which pull straight from HBM.
+@pytest.mark.parametrize("alg,layout,storage", [
+    ("nptl", "collision", "f64"), ("nptl", "collision", "f32"), ("lptl", "collision", "f16-shifted"),
+    ("nptl", "collision", "bf16-shifted"), ("nptl", "bundle", "f64"), ("lptl", "bundle", "f32")])
+@pytest.mark.parametrize("w,h", [(37, 21), (130, 43), (3, 3), (65, 9)])
+def test_staged_kernels_equal_split_steps_on_ragged_shapes(alg, layout, storage, w, h):
+    doc = pa.settings(alg, layout, inner_h=w, inner_v=h, scenario="circle" if min(w, h) > 8 else "walls",
+                      work_group_size=128, time_steps=1, tau=0.7, rho_in=1.05, storage=storage)
+    with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
+        keep = _fluid_slots(fused)
+        fused.run(23)
+        split.run_split(23)
+        np.testing.assert_array_equal(split.populations()[keep], fused.populations()[keep])
+        for a, b in zip(split.macros(), fused.macros()):
+            np.testing.assert_array_equal(a, b)
+
+
 def _fluid_slots(lat):
     """Population indices (layout order of lbm_get_populations) of the fluid nodes."""
     N = lat.N
