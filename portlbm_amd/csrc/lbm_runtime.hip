@@ -833,28 +833,36 @@ struct Impl
                 hipLaunchKernelGGL((k_step_nptl_staged<L, S>), grid, dim3(NPTL_STAGED_BLOCK * rows), 0, c->stream, p);
             }
         };
+        auto launch_rows = [&]() {
+            constexpr uint32_t R = nptl_rows<S>();
+            dim3 grid((c->Hx - 2u + NPTL_BLOCK_X - 1u) / NPTL_BLOCK_X, (c->Vx - 2u + R - 1u) / R);
+            hipLaunchKernelGGL((k_step_nptl<L, S>), grid, dim3(NPTL_BLOCK_X), 0, c->stream, p);
+        };
         switch (c->desc.algorithm)
         {
         case LBM_ALG_NPTL:
         {
             if constexpr (nptl_staged<L, S>())
-            {
                 launch_staged();
-                break;
-            }
-            constexpr uint32_t R = nptl_rows<S>();
-            dim3 grid((c->Hx - 2u + NPTL_BLOCK_X - 1u) / NPTL_BLOCK_X, (c->Vx - 2u + R - 1u) / R);
-            hipLaunchKernelGGL((k_step_nptl<L, S>), grid, dim3(NPTL_BLOCK_X), 0, c->stream, p);
+            else
+                launch_rows();
             break;
         }
         case LBM_ALG_LPTL:
         {
-            // Same two-lattice update on the same placement; where the staged
-            // kernel is faster (collision, FP64/FP32 bundle) lptl runs it too
-            // and the work-group size of the settings does not apply.
+            // Same two-lattice update on the same placement; where nptl's
+            // kernels are faster -- the staged one (collision, FP64/FP32
+            // bundle) and the two-rows-per-thread one (16-bit storage) --
+            // lptl runs them too and the work-group size of the settings
+            // does not apply.
             if constexpr (nptl_staged<L, S>())
             {
                 launch_staged();
+                break;
+            }
+            if constexpr (sizeof(typename Storage<S>::T) == 2)
+            {
+                launch_rows();
                 break;
             }
             const uint32_t b = c->wg;

@@ -27,7 +27,8 @@ def _doc(alg, layout, storage="f64", rounding="nearest", scenario="circle"):
     ("nsol", "collision", "f64", "nearest"), ("nsol", "bundle", "f16-shifted", "stochastic"),
     ("lptl", "stream", "bf16-shifted", "nearest"), ("nptl", "collision", "f64", "nearest"),
     ("nptl", "collision", "f16-shifted", "stochastic"), ("lptl", "collision", "bf16-shifted", "nearest"),
-    ("nptl", "bundle", "f32", "nearest")])
+    ("nptl", "bundle", "f32", "nearest"), ("lptl", "bundle", "f16-shifted", "nearest"),
+    ("nptl", "stream", "f16-shifted", "stochastic")])
 def test_split_steps_equal_fused_steps(alg, layout, storage, rounding):
     doc = _doc(alg, layout, storage, rounding)
     with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
