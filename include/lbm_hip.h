@@ -128,7 +128,8 @@ const char *lbm_last_error(void);
 /* Largest work-group the device accepts (1024 on gfx950).
  * Replaces sycl::info::device::max_work_group_size (simulation.cpp:443,487). */
 int lbm_device_max_work_group_size(uint32_t *out);
-/* Name of the current device ("<marketing name> (<gfx arch>)"), NUL-terminated,
+/* Name of the current device ("<marketing name> (<gfx arch>)", or the gfx arch
+ * alone when the runtime reports no marketing name), NUL-terminated,
  * truncated to len - 1 characters.  Replaces sycl::info::device::name
  * (src/lbm/benchmark/benchmark.cpp:77). */
 int lbm_device_name(char *buf, size_t len);

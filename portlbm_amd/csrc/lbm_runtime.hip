@@ -1073,7 +1073,12 @@ int lbm_device_name(char *buf, size_t len)
     LBM_HIP_TRY(hipGetDevice(&dev));
     hipDeviceProp_t prop{};
     LBM_HIP_TRY(hipGetDeviceProperties(&prop, dev));
-    const std::string name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    // the marketing name can be empty (e.g. in containers); then the architecture alone
+    std::string market(prop.name);
+    while (!market.empty() && market.back() == ' ')
+        market.pop_back();
+    const std::string arch(prop.gcnArchName);
+    const std::string name = market.empty() ? arch : market + " (" + arch + ")";
     const size_t n = std::min(len - 1, name.size());
     std::memcpy(buf, name.data(), n);
     buf[n] = '\0';
