@@ -110,11 +110,12 @@ constexpr int NPTL_BLOCK_X = 256;
 // A thread updates nptl_rows<S>() vertically adjacent nodes, issuing all their
 // pulls before any arithmetic: with 2-byte populations one row per wave keeps
 // too few bytes in flight to cover HBM latency (two rows: +5.5 % on the
-// 8192 x 2048 porous channel in FP16; FP32 and FP64 lose 2-4 % and keep one).
+// 8192 x 2048 porous channel in FP16, three: +1 % more, four: -1 %; FP32 and
+// FP64 lose 2-4 % with two and keep one).
 template <int S>
 constexpr uint32_t nptl_rows()
 {
-    return sizeof(typename Storage<S>::T) == 2 ? 2u : 1u;
+    return sizeof(typename Storage<S>::T) == 2 ? 3u : 1u;
 }
 
 template <int L, int S>
