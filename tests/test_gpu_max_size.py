@@ -1,5 +1,5 @@
 """A lattice close to the 32-bit population-index limit (9 x places < 2^32):
-lptl, FP64, stream layout, 70 000 000 x 4 inner nodes (420 M places, 9 x the
+lptl, FP64, each layout, 70 000 000 x 4 inner nodes (420 M places, 9 x the
 direction stride = 3.8e9; 61 GB of lattices).  Influence travels one column
 per step, so after 20 steps the first and the last 100 columns must equal a
 200 x 4 channel's bit for bit: the inlet side checks the smallest indices, the
@@ -15,8 +15,8 @@ WIDE = 70_000_000
 STEPS = 20
 
 
-def _doc(width):
-    return pa.settings("lptl", "stream", inner_h=width, inner_v=4, scenario="walls", work_group_size=256,
+def _doc(width, layout="stream"):
+    return pa.settings("lptl", layout, inner_h=width, inner_v=4, scenario="walls", work_group_size=256,
                        time_steps=STEPS, tau=0.6, rho_in=1.05, rho_out=1.0, storage="f64")
 
 
@@ -32,22 +32,23 @@ def _channel(width):
     return ph.ravel()
 
 
-def _lattice(width):
-    lat = pa.Lattice(pa.desc_from_settings(_doc(width)))
+def _lattice(width, layout):
+    lat = pa.Lattice(pa.desc_from_settings(_doc(width, layout)))
     lat.set_phase(_channel(width))
     lat.init()
     return lat
 
 
-def test_lattice_near_the_index_limit_matches_a_small_one_at_both_ends():
-    desc = pa.desc_from_settings(_doc(WIDE))
+@pytest.mark.parametrize("layout", ["stream", "collision", "bundle"])
+def test_lattice_near_the_index_limit_matches_a_small_one_at_both_ends(layout):
+    desc = pa.desc_from_settings(_doc(WIDE, layout))
     places = (63 + desc.expanded_horizontal_nodes + 63) // 64 * 64 * desc.expanded_vertical_nodes
     assert 9 * places > 0.85 * 2**32  # close to the limit, and accepted
-    with _lattice(200) as small:
+    with _lattice(200, layout) as small:
         small.run(STEPS)
         ref = [a.reshape(4, 200) for a in small.macros()]
     print(f"creating {WIDE} x 4 (9 x places = {9 * places:.3e})", flush=True)
-    with _lattice(WIDE) as big:
+    with _lattice(WIDE, layout) as big:
         big.run(STEPS)
         got = [a.reshape(4, WIDE) for a in big.macros()]
     for a, b in zip(got, ref):
