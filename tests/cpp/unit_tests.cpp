@@ -4,8 +4,10 @@
 #include <cmath>
 #include <cstdio>
 #include <filesystem>
+#include <fstream>
 #include <functional>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -525,6 +527,80 @@ void benchmark_settings_file()
     CHECK(threw);
 }
 TEST_CASE("Benchmark: settings/benchmark.json", benchmark_settings_file);
+
+// ---- golden known answers ----------------------------------------------------
+// tests/golden/reference_kats.json: the access and power-function known answers
+// of the reference's tests/test_access.cpp as data (inputs, expected outputs,
+// reference file:line), checked against the C++ surface.
+void golden_known_answers()
+{
+    if (g_settings_path.empty())
+        return;
+    const std::filesystem::path file = std::filesystem::path(g_settings_path).parent_path().parent_path() /
+                                       "tests" / "golden" / "reference_kats.json";
+    CHECK(std::filesystem::exists(file));
+    std::ifstream in(file);
+    std::stringstream text;
+    text << in.rdbuf();
+    using lbm::file_interaction::JsonValue;
+    const JsonValue doc = JsonValue::parse(text.str());
+    const auto u = [](const JsonValue &c, const char *key) { return static_cast<unsigned int>(c.at(key).as_unsigned(key)); };
+    int cases = 0;
+    for (const JsonValue &c : doc.at("accessor").items())
+    {
+        const std::string &layout = c.at("layout").as_string("layout");
+        const unsigned int n = u(c, "node"), d = u(c, "dir"), t = u(c, "total"), e = u(c, "expect");
+        if (layout == "collision")
+            CHECK(CollisionAccessor::at(n, d, t) == e);
+        else if (layout == "stream")
+            CHECK(StreamAccessor::at(n, d, t) == e);
+        else
+            CHECK(BundleAccessor::at(n, d, t) == e);
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("node_index").items())
+    {
+        CHECK(get_node_index(u(c, "x"), u(c, "y"), u(c, "H")) == u(c, "expect"));
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("neighbor").items())
+    {
+        CHECK(get_neighbor(u(c, "node"), u(c, "dir"), u(c, "H")) == u(c, "expect"));
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("result_index_xy").items())
+    {
+        CHECK(get_result_index(u(c, "x"), u(c, "y"), u(c, "H")) == u(c, "expect"));
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("result_index_series").items())
+    {
+        CHECK(get_result_index(u(c, "x"), u(c, "y"), u(c, "H"), u(c, "domain_count"), u(c, "step")) == u(c, "expect"));
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("result_index_linear").items())
+    {
+        CHECK(get_result_index(u(c, "node"), u(c, "H")) == u(c, "expect"));
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("is_power_of_4").items())
+    {
+        CHECK(is_power_of_4(u(c, "x")) == c.at("expect").as_bool("expect"));
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("which_power_of_4").items())
+    {
+        CHECK(which_power_of_4(u(c, "x")) == u(c, "expect"));
+        ++cases;
+    }
+    for (const JsonValue &c : doc.at("which_power_of_2").items())
+    {
+        CHECK(which_power_of_2(u(c, "x")) == u(c, "expect"));
+        ++cases;
+    }
+    CHECK(cases >= 70);
+}
+TEST_CASE("Golden: the reference's known answers (tests/golden/reference_kats.json)", golden_known_answers);
 
 }  // namespace
 

@@ -8,6 +8,8 @@ outlet algebra, the Poiseuille regression of the reference, cross-algorithm
 equality, and brute-force recomputation on small lattices.
 """
 import ctypes
+import json
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -47,6 +49,30 @@ EXTENTS = [
 def test_domain_extents_known_answers(alg, w, h, wg, expect):
     e = ob.domain(alg, w + 2, h + 2, wg)
     assert (e["Hx"], e["Vx"], e["N"]) == expect
+
+
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_kats.json").read_text())
+DOMAIN_FIELDS = {"horizontal_nodes": "Hx", "vertical_nodes": "Vx", "total_node_count": "N",
+                 "subdomain_horizontal_nodes": "sub_h", "subdomain_vertical_nodes": "sub_v",
+                 "subdomain_count_horizontal": "cnt_h", "subdomain_count_vertical": "cnt_v"}
+
+
+@pytest.mark.parametrize("case", GOLDEN["domain"], ids=lambda c: f"{c['algorithm']}-wg{c['wg']}-{c['source']}")
+def test_domain_golden_cases_oracle_and_host_library(case):
+    """core::Domain known answers of the reference's tests/test_domain.cpp, for the
+    oracle and for the host library's Domain (the lattice extents the C ABI is given)."""
+    H, V = case["inner_h"] + 2, case["inner_v"] + 2
+    doc = pa.settings(case["algorithm"], "stream", inner_h=case["inner_h"], inner_v=case["inner_v"],
+                      work_group_size=case["wg"])
+    host = pa.domain_extents(doc)
+    for impl in (ob.domain(case["algorithm"], H, V, case["wg"]), host):
+        assert impl["N"] == impl["Hx"] * impl["Vx"]
+        for field, value in case.get("equal", {}).items():
+            assert impl[DOMAIN_FIELDS[field]] == value, (field, impl)
+        for field, value in case.get("at_least", {}).items():
+            assert impl[DOMAIN_FIELDS[field]] >= value, (field, impl)
+        if "subdomain_product" in case:
+            assert impl["sub_h"] * impl["sub_v"] == case["subdomain_product"]
 
 
 def test_domain_closed_forms_and_reference_unit_cases():
