@@ -22,7 +22,8 @@
  *     npol/npol.hpp:180-201, nsol/nsol.hpp:184-205
  *     (EmplaceBounceBackKernel, OutletUpdateKernel, StreamCollideKernel and
  *      the buffer-copy kernels of every algorithm are fused into one launch
- *      per step plus, for the one-lattice schemes, one halo refresh)
+ *      per step; the one-lattice schemes with tiles wider than 64 nodes, and
+ *      their FP64 collision layout, add one buffer refresh)
  *   SYCLAlgorithm::copy_macroscopic_observables_to_cpu   lbm_get_macros
  *     include/lbm/execution/sycl_algorithm.hpp:90-104
  *   the same copies enqueued after each frame step       lbm_run_frames /
@@ -213,7 +214,10 @@ int lbm_get_frame(lbm_ctx *ctx, double *density, double *x_velocity, double *y_v
 
 /* Current source lattice: 9 * expanded node count values in the context's
  * layout order (core/access.hpp:117-171), widened to double; shifted
- * storage is returned as f_i (the weight added back). */
+ * storage is returned as f_i (the weight added back).  npol/nsol with tiles
+ * at most 64 nodes wide keep the buffer copies of a step outside the lattice,
+ * so the lattice's buffer rows/columns hold what they held when last
+ * refreshed (a split step refreshes them); every other node is current. */
 int lbm_get_populations(lbm_ctx *ctx, double *values, size_t count);
 int lbm_set_populations(lbm_ctx *ctx, const double *values, size_t count);
 

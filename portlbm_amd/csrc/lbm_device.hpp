@@ -217,6 +217,11 @@ struct StepParams
     uint32_t write_macros;
     uint32_t sr;          // stochastic-rounding seed of this step (0: round to nearest)
     uint32_t nan_protection;
+    // npol/nsol tile sweep: element offsets, from the lattice base, of the
+    // copies of tile-edge populations that the buffer rows/columns carry,
+    // read this step and written for the next
+    uint32_t side_rows_in, side_cols_in, side_rows_out, side_cols_out;
+    uint32_t cnt_h, cnt_v;  // tiles per row / column
 };
 
 // NaN clamp of WITH_NAN_PROTECTION (nptl_kernels.hpp:413-432): NaN or values
@@ -361,6 +366,47 @@ __device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t
             Storage<S>::store_nt(dst, at<L>(p, uint32_t(d), Sd), f[d], d);
         else
             Storage<S>::store(dst, at<L>(p, uint32_t(d), Sd), f[d], d);
+    }
+}
+
+// The bits store_node writes for a node's nine populations (same rounding,
+// same random stream), for a caller that also writes them elsewhere.
+template <int S>
+__device__ __forceinline__ void node_bits(const typename Storage<S>::C f[9], uint32_t p, uint32_t sr,
+                                          typename Storage<S>::T t[9])
+{
+    if constexpr (S == LBM_STORE_F16_SHIFTED || S == LBM_STORE_BF16_SHIFTED)
+    {
+        if (sr)
+        {
+            constexpr int drop = S == LBM_STORE_F16_SHIFTED ? 19 : 16;
+            uint32_t h = mix32(sr ^ p);
+#pragma unroll
+            for (int d = 0; d < 9; ++d)
+            {
+                h = h * 747796405u + 2891336453u;
+                const uint32_t r = h >> drop;
+                if constexpr (S == LBM_STORE_F16_SHIFTED)
+                    t[d] = f32_to_f16_sr(f[d] - weight<float>(d), r);
+                else
+                    t[d] = f32_to_bf16_sr(f[d] - weight<float>(d), r);
+            }
+            return;
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+        {
+            if constexpr (S == LBM_STORE_F16_SHIFTED)
+                t[d] = _Float16(f[d] - weight<float>(d));
+            else
+                t[d] = f32_to_bf16(f[d] - weight<float>(d));
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            t[d] = f[d];
     }
 }
 

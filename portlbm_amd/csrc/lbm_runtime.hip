@@ -58,6 +58,13 @@ struct lbm_ctx
     bool buffered;
     bool two_lattice;
     bool outlet_active;
+    // npol/nsol tiles at most one wave wide run k_step_tile_sweep, which keeps
+    // the buffer copies in side arrays (two of each kind, used in turn)
+    bool sweep = false;
+    uint32_t side_rows[2] = {0, 0};  // element offsets from lat[0]
+    uint32_t side_cols[2] = {0, 0};
+    uint32_t side_parity = 0;
+    bool side_valid = false;  // false: rebuild them from the lattice before the next sweep
     size_t elem;          // bytes per stored population
     void *lat[2] = {nullptr, nullptr};
     int cur = 0;          // index of the source lattice
@@ -523,6 +530,238 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
         store_node<L, S>(lat, pn, p.S, f, p.sr);
 }
 
+// npol and nsol with tiles at most one wave wide: one wave per tile, walking
+// up the tile R = 64 / sub_h node rows at a time, with no barrier and no
+// buffer-refresh launches.
+//
+// The buffer rows and columns of the lattice carry copies of tile-edge
+// populations from the previous step (general/buffered.hpp:74-93,133-152).
+// Here those copies live in compact side arrays, two of each kind used in
+// turn: a step reads its neighbours' previous edge values from one and each
+// tile writes its own new edge values to the other, so tiles need no order
+// among themselves.  A pull through a buffer node takes what the two refresh
+// passes would have put there, corners included (rows first, then columns:
+// a corner carries its diagonal neighbour's population).  These steps leave
+// the lattice's own buffer nodes as they are; only the split passes read
+// those, and they refresh them first.
+//
+// Inside a tile a row group loads everything it pulls before it stores.  It
+// pulls from its own rows, from rows above it (not written yet) and from the
+// side arrays, except populations 6, 7, 8 of its bottom row, which come from
+// the previous group's top row: that group overwrote them, so it loaded its
+// own old values first and hands them down across lanes (the load that a
+// hand-down replaces is issued all the same and its value dropped, so that
+// the pulls of a group are one batch of loads).
+//
+// Side-array layouts (elements of the storage type), boundary k between
+// tiles k and k + 1:
+//   rows: [k < cnt_v - 1][j < 6][x < Hx], yb = (k + 1)(sub_v + 1):
+//         j < 3 population j of (x, yb + 1), j >= 3 population j + 3 of (x, yb - 1)
+//   cols: [k < cnt_h - 1][j < 6][y < Vx], xb = (k + 1)(sub_h + 1):
+//         j < 3 population 3j + 2 of (xb - 1, y), j >= 3 population 3(j - 3) of (xb + 1, y)
+constexpr uint32_t TILE_SWEEP_BLOCK = 256;  // four tiles per block
+constexpr uint32_t TILE_SWEEP_MAX_WIDTH = 64;
+
+// Does the 64-byte segment of a lane's node hold a node of `pred`?  Lanes
+// lo..hi-1 hold consecutive places of one row; E elements per node are
+// contiguous in an array (stream 1, bundle 3, collision 9) whose base is
+// segment aligned.  Exact where the lane-group test of segment_has assumes
+// rows that start on a segment.
+template <class T, uint32_t E>
+__device__ __forceinline__ bool row_segment_has(bool pred, uint32_t pn, int lane, int lo, int hi)
+{
+    constexpr uint32_t G = 64u / sizeof(T);
+    const unsigned long long m = __ballot(pred);
+    const uint32_t s0 = (E * pn) / G, s1 = (E * pn + E - 1u) / G;
+    const uint32_t q0 = (s0 * G) / E, q1 = (s1 * G + G - 1u) / E;  // first and last node in them
+    const int a = max(lo, lane - int(pn - q0));
+    const int b = min(hi - 1, lane + int(q1 - pn));
+    const int n = b - a + 1;
+    const unsigned long long span = (n >= 64 ? ~0ull : ((1ull << n) - 1ull)) << a;
+    return (m & span) != 0ull;
+}
+
+template <int L, int S>
+__global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams p, uint32_t tiles)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    constexpr uint32_t E = L == LBM_LAYOUT_STREAM ? 1u : (L == LBM_LAYOUT_BUNDLE ? 3u : 9u);
+    T *lat = static_cast<T *>(p.dst);  // the side arrays follow the lattice in its allocation
+    const uint32_t tile = blockIdx.x * (TILE_SWEEP_BLOCK / 64u) + threadIdx.x / 64u;
+    if (tile >= tiles)
+        return;  // whole waves
+    const uint32_t sh = p.sub_h, sv = p.sub_v, P = p.P, Sd = p.S, Hx = p.Hx, Vx = p.Vx;
+    const uint32_t R = 64u / sh;  // node rows per group
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = lane / sh, i = lane - r * sh;
+    const uint32_t by = tile / p.cnt_h, bx = tile - by * p.cnt_h;
+    const uint32_t x = bx * (sh + 1u) + 1u + i;
+    const uint32_t xi = x - x / (sh + 1u);  // unexpanded column
+    const uint32_t y00 = by * (sv + 1u) + 1u;
+    const int lo = int(r * sh), hi = int(r * sh + sh);
+    const bool buf_left = i == 0u && bx > 0u;                // x - 1 is a buffer column
+    const bool buf_right = i == sh - 1u && bx + 1u < p.cnt_h;  // x + 1 is one
+    // lanes of a group's top row that the next group's bottom-row pulls of
+    // 6 (from x + 1), 7 (x) and 8 (x - 1) read
+    const uint32_t top = (R - 1u) * sh;
+    const int src6 = int(top + min(i + 1u, sh - 1u)), src7 = int(top + i), src8 = int(top + (i ? i - 1u : 0u));
+
+    // this lane's node code in the group starting at tile row y0 (0 outside the tile)
+    auto code_at = [&](uint32_t y0) -> uint32_t {
+        const uint32_t row = y0 + r;
+        return (r < R && row < sv) ? uint32_t(p.code[place(x, y00 + row, P, p.X0)]) : 0u;
+    };
+    // the pulls of the group starting at y0
+    auto gather = [&](uint32_t y0, uint32_t code, C f[9]) {
+        if (!(code & LBM_CODE_FLUID))
+            return;
+        const uint32_t row = y0 + r, y = y00 + row;
+        const uint32_t pn = place(x, y, P, p.X0);
+        const bool buf_below = row == 0u && by > 0u;             // y - 1 is a buffer row
+        const bool buf_above = row == sv - 1u && by + 1u < p.cnt_v;  // y + 1 is one
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+        {
+            const int cx = cx_of(d), cy = cy_of(d);
+            const bool bb = (code >> d) & 1u;
+            const bool cb = (cx == 1 && buf_left) || (cx == -1 && buf_right);
+            const bool rb = (cy == 1 && buf_below) || (cy == -1 && buf_above);
+            uint32_t idx;
+            if (bb)
+                idx = at<L>(pn, uint32_t(8 - d), Sd);
+            else if (rb)
+            {
+                const uint32_t k = cy == 1 ? by - 1u : by;
+                const uint32_t xs = uint32_t(int(x) - cx - (cb ? cx : 0));
+                const uint32_t j = cy == -1 ? uint32_t(d) : uint32_t(d - 3);
+                idx = p.side_rows_in + (k * 6u + j) * Hx + xs;
+            }
+            else if (cb)
+            {
+                const uint32_t k = cx == 1 ? bx - 1u : bx;
+                const uint32_t j = cx == 1 ? uint32_t(d - 2) / 3u : 3u + uint32_t(d) / 3u;
+                idx = p.side_cols_in + (k * 6u + j) * Vx + uint32_t(int(y) - cy);
+            }
+            else
+                idx = at<L>(uint32_t(int32_t(pn) - cy * int32_t(P) - cx), uint32_t(d), Sd);
+            f[d] = Storage<S>::load(lat, idx, bb ? 8 - d : d);
+        }
+    };
+    // own old populations 6, 7, 8 of the group's top row, when a group follows
+    auto keep_top = [&](uint32_t y0, uint32_t code, C &c6, C &c7, C &c8) {
+        if (r == R - 1u && y0 + R < sv && code && !(code & LBM_CODE_SOLID))
+        {
+            const uint32_t pn = place(x, y00 + y0 + r, P, p.X0);
+            c6 = Storage<S>::load(lat, at<L>(pn, 6u, Sd), 6);
+            c7 = Storage<S>::load(lat, at<L>(pn, 7u, Sd), 7);
+            c8 = Storage<S>::load(lat, at<L>(pn, 8u, Sd), 8);
+        }
+    };
+
+    C c6 = C(0), c7 = C(0), c8 = C(0);  // old 6, 7, 8 of the previous group's top row
+    for (uint32_t y0 = 0; y0 < sv; y0 += R)
+    {
+        const uint32_t code = code_at(y0);
+        const C k6 = __shfl(c6, src6, 64), k7 = __shfl(c7, src7, 64), k8 = __shfl(c8, src8, 64);
+        keep_top(y0, code, c6, c7, c8);
+        C f[9] = {};
+        gather(y0, code, f);
+
+        const uint32_t row = y0 + r, y = y00 + row;
+        const uint32_t pn = place(x, y, P, p.X0);
+        const bool fluid = code & LBM_CODE_FLUID;
+        const bool fill = (code & LBM_CODE_SOLID) && row_segment_has<T, E>(fluid, pn, int(lane), lo, hi);
+        if (fluid)
+        {
+            if (y0 != 0u && r == 0u)
+            {
+#pragma unroll
+                for (int d = 6; d < 9; ++d)
+                {
+                    const int sx = int(i) - cx_of(d);
+                    if (!((code >> d) & 1u) && sx >= 0 && sx < int(sh))
+                        f[d] = d == 6 ? k6 : (d == 7 ? k7 : k8);
+                }
+            }
+            finish_node<L, S>(p, code, xi, y - y / (sv + 1u), f);
+        }
+        // keep_top's loads have returned before any lane overwrites what they read
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (fluid || fill)
+        {
+            T t[9];
+            node_bits<S>(f, pn, p.sr, t);
+#pragma unroll
+            for (int d = 0; d < 9; ++d)
+            {
+                if constexpr (L == LBM_LAYOUT_STREAM)
+                    __builtin_nontemporal_store(t[d], lat + at<L>(pn, uint32_t(d), Sd));
+                else
+                    lat[at<L>(pn, uint32_t(d), Sd)] = t[d];
+            }
+            if (fluid)
+            {
+                if (buf_right)
+                    for (uint32_t j = 0; j < 3; ++j)
+                        lat[p.side_cols_out + (bx * 6u + j) * Vx + y] = t[3u * j + 2u];
+                if (buf_left)
+                    for (uint32_t j = 0; j < 3; ++j)
+                        lat[p.side_cols_out + ((bx - 1u) * 6u + 3u + j) * Vx + y] = t[3u * j];
+                if (row == sv - 1u && by + 1u < p.cnt_v)
+                    for (uint32_t j = 0; j < 3; ++j)
+                        lat[p.side_rows_out + (by * 6u + 3u + j) * Hx + x] = t[6u + j];
+                if (row == 0u && by > 0u)
+                    for (uint32_t j = 0; j < 3; ++j)
+                        lat[p.side_rows_out + ((by - 1u) * 6u + j) * Hx + x] = t[j];
+            }
+        }
+    }
+}
+
+// The side arrays of the tile sweep for both step parities, from the
+// lattice's real nodes: what k_refresh_rows / k_refresh_cols would copy into
+// the buffer nodes now.
+template <int L, class T>
+__global__ void k_prime_side_rows(const T *lat, T *out0, T *out1, uint32_t Hx, uint32_t P, uint32_t X0, uint32_t Sd,
+                                  uint32_t sub_v)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= Hx)
+        return;
+    const uint32_t k = blockIdx.y;
+    const uint32_t yb = (k + 1u) * (sub_v + 1u);
+    const uint32_t up = place(x, yb + 1u, P, X0), down = place(x, yb - 1u, P, X0);
+    for (uint32_t j = 0; j < 3; ++j)
+    {
+        const T a = lat[at<L>(up, j, Sd)], b = lat[at<L>(down, 6u + j, Sd)];
+        out0[(k * 6u + j) * Hx + x] = a;
+        out1[(k * 6u + j) * Hx + x] = a;
+        out0[(k * 6u + 3u + j) * Hx + x] = b;
+        out1[(k * 6u + 3u + j) * Hx + x] = b;
+    }
+}
+
+template <int L, class T>
+__global__ void k_prime_side_cols(const T *lat, T *out0, T *out1, uint32_t Vx, uint32_t P, uint32_t X0, uint32_t Sd,
+                                  uint32_t sub_h)
+{
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= Vx)
+        return;
+    const uint32_t k = blockIdx.y;
+    const uint32_t xb = (k + 1u) * (sub_h + 1u);
+    const uint32_t left = place(xb - 1u, y, P, X0), right = place(xb + 1u, y, P, X0);
+    for (uint32_t j = 0; j < 3; ++j)
+    {
+        const T a = lat[at<L>(left, 3u * j + 2u, Sd)], b = lat[at<L>(right, 3u * j, Sd)];
+        out0[(k * 6u + j) * Vx + y] = a;
+        out1[(k * 6u + j) * Vx + y] = a;
+        out0[(k * 6u + 3u + j) * Vx + y] = b;
+        out1[(k * 6u + 3u + j) * Vx + y] = b;
+    }
+}
+
 // Buffer rows: directions 0..2 from the row above, 6..8 from the row below
 // (general/buffered.hpp:74-93).  Covers the full expanded width, ghost
 // columns included.
@@ -534,12 +773,21 @@ __global__ void k_refresh_rows(T *lat, uint32_t Hx, uint32_t P, uint32_t X0, uin
         return;
     const uint32_t yb = (blockIdx.y + 1u) * (sub_v + 1u);
     const uint32_t n = place(x, yb, P, X0);
+    // sources and destinations never overlap: all six loads go out before
+    // the first store instead of one round trip per population
+    T v[6];
 #pragma unroll
     for (uint32_t d = 0; d < 3; ++d)
-        lat[at<L>(n, d, Sd)] = lat[at<L>(n + P, d, Sd)];
+    {
+        v[d] = lat[at<L>(n + P, d, Sd)];
+        v[3 + d] = lat[at<L>(n - P, 6u + d, Sd)];
+    }
 #pragma unroll
-    for (uint32_t d = 6; d < 9; ++d)
-        lat[at<L>(n, d, Sd)] = lat[at<L>(n - P, d, Sd)];
+    for (uint32_t d = 0; d < 3; ++d)
+    {
+        lat[at<L>(n, d, Sd)] = v[d];
+        lat[at<L>(n, 6u + d, Sd)] = v[3 + d];
+    }
 }
 
 // Buffer columns: directions 2, 5, 8 from the left, 0, 3, 6 from the right
@@ -553,12 +801,19 @@ __global__ void k_refresh_cols(T *lat, uint32_t Vx, uint32_t P, uint32_t X0, uin
         return;
     const uint32_t xb = (blockIdx.y + 1u) * (sub_h + 1u);
     const uint32_t n = place(xb, y, P, X0);
+    T v[6];  // all loads first, as in k_refresh_rows
 #pragma unroll
-    for (uint32_t d = 2; d < 9; d += 3)
-        lat[at<L>(n, d, Sd)] = lat[at<L>(n - 1u, d, Sd)];
+    for (uint32_t k = 0; k < 3; ++k)
+    {
+        v[k] = lat[at<L>(n - 1u, 3u * k + 2u, Sd)];
+        v[3 + k] = lat[at<L>(n + 1u, 3u * k, Sd)];
+    }
 #pragma unroll
-    for (uint32_t d = 0; d < 9; d += 3)
-        lat[at<L>(n, d, Sd)] = lat[at<L>(n + 1u, d, Sd)];
+    for (uint32_t k = 0; k < 3; ++k)
+    {
+        lat[at<L>(n, 3u * k + 2u, Sd)] = v[k];
+        lat[at<L>(n, 3u * k, Sd)] = v[3 + k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -820,6 +1075,12 @@ struct Impl
         p.nan_protection = c->desc.nan_protection ? 1u : 0u;
         // per-step seed, never 0 (0 selects round-to-nearest)
         p.sr = c->desc.rounding == LBM_ROUND_STOCHASTIC ? (step_seed(c->iteration) | 1u) : 0u;
+        p.cnt_h = c->cnt_h;
+        p.cnt_v = c->cnt_v;
+        p.side_rows_in = c->side_rows[c->side_parity];
+        p.side_cols_in = c->side_cols[c->side_parity];
+        p.side_rows_out = c->side_rows[c->side_parity ^ 1];
+        p.side_cols_out = c->side_cols[c->side_parity ^ 1];
         return p;
     }
 
@@ -838,6 +1099,15 @@ struct Impl
             constexpr uint32_t R = nptl_rows<S>();
             dim3 grid((c->Hx - 2u + NPTL_BLOCK_X - 1u) / NPTL_BLOCK_X, (c->Vx - 2u + R - 1u) / R);
             hipLaunchKernelGGL((k_step_nptl<L, S>), grid, dim3(NPTL_BLOCK_X), 0, c->stream, p);
+        };
+        auto launch_sweep = [&]() {
+            if (!c->side_valid)
+                prime(c);
+            const uint32_t tiles = c->cnt_h * c->cnt_v;
+            const uint32_t per_block = TILE_SWEEP_BLOCK / 64u;
+            hipLaunchKernelGGL((k_step_tile_sweep<L, S>), dim3((tiles + per_block - 1u) / per_block),
+                               dim3(TILE_SWEEP_BLOCK), 0, c->stream, p, tiles);
+            c->side_parity ^= 1;
         };
         switch (c->desc.algorithm)
         {
@@ -872,12 +1142,22 @@ struct Impl
             break;
         }
         case LBM_ALG_NPOL:
+            if (c->sweep)
+            {
+                launch_sweep();
+                break;
+            }
             refresh(c);
             hipLaunchKernelGGL((k_step_npol<L, S>), dim3(c->cnt_h, c->cnt_v), dim3(c->sub_h, c->sub_v), 0,
                                c->stream, p);
             break;
         case LBM_ALG_NSOL:
         {
+            if (c->sweep)
+            {
+                launch_sweep();
+                break;
+            }
             refresh(c);
             const size_t lds = 4u * size_t(c->sub_v + 2u) * (c->sub_h + 4u) * sizeof(C);
             hipLaunchKernelGGL((k_step_nsol<L, S>), dim3(c->cnt_h, c->cnt_v), dim3(c->sub_h + 2u, c->sub_v + 1u),
@@ -896,6 +1176,7 @@ struct Impl
     // the step (lattice swap, iteration count).
     static int split(lbm_ctx *c, int pass, double *buf)
     {
+        c->side_valid = false;  // the passes change the lattice behind the side arrays
         const StepParams p = params(c, true);
         const uint32_t PV = c->P * c->Vx;
         const dim3 grid((PV + 255u) / 256u), block(256);
@@ -934,6 +1215,21 @@ struct Impl
         if (c->cnt_h > 1)
             hipLaunchKernelGGL((k_refresh_cols<L, T>), dim3((c->Vx + 255u) / 256u, c->cnt_h - 1u), dim3(256), 0,
                                c->stream, lat, c->Vx, c->P, c->X0, c->Sd, c->sub_h);
+    }
+
+    // Side arrays of the tile sweep from the current lattice (both parities).
+    static void prime(lbm_ctx *c)
+    {
+        T *lat = static_cast<T *>(c->lat[c->cur]);
+        if (c->cnt_v > 1)
+            hipLaunchKernelGGL((k_prime_side_rows<L, T>), dim3((c->Hx + 255u) / 256u, c->cnt_v - 1u), dim3(256), 0,
+                               c->stream, lat, lat + c->side_rows[0], lat + c->side_rows[1], c->Hx, c->P, c->X0, c->Sd,
+                               c->sub_v);
+        if (c->cnt_h > 1)
+            hipLaunchKernelGGL((k_prime_side_cols<L, T>), dim3((c->Vx + 255u) / 256u, c->cnt_h - 1u), dim3(256), 0,
+                               c->stream, lat, lat + c->side_cols[0], lat + c->side_cols[1], c->Vx, c->P, c->X0, c->Sd,
+                               c->sub_h);
+        c->side_valid = true;
     }
 
     static Geom geom(const lbm_ctx *c) { return Geom{c->Hx, c->N, c->P, c->X0, c->Sd}; }
@@ -1230,7 +1526,31 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     // aligned 16-byte vectors, up to 15 bytes past the last place of the top
     // ghost row
     const size_t lat_bytes = c->lattice_elems * c->elem;
-    CREATE_TRY(hipMalloc(&c->lat[0], lat_bytes + 64));
+    // npol/nsol tiles at most one wave wide: the tile sweep's side arrays go
+    // after the lattice in the same allocation (32-bit element offsets), two
+    // of each kind, each starting on 256 bytes
+    // (FP64 in the collision layout excepted: its strided 72-byte nodes ran
+    // faster with the block-per-tile kernels and the refresh launches)
+    size_t side_bytes = 0;
+    if (c->buffered && c->sub_h <= TILE_SWEEP_MAX_WIDTH &&
+        !(desc->layout == LBM_LAYOUT_COLLISION && c->elem == 8))
+    {
+        const size_t align = 256 / c->elem;
+        auto round_up = [&](size_t n) { return (n + align - 1) / align * align; };
+        const size_t rows = round_up(size_t(c->cnt_v - 1u) * 6u * c->Hx);
+        const size_t cols = round_up(size_t(c->cnt_h - 1u) * 6u * c->Vx);
+        const size_t first = round_up(c->lattice_elems + 64 / c->elem);
+        if (first + 2 * (rows + cols) < (1ull << 32))
+        {
+            c->sweep = true;
+            c->side_rows[0] = uint32_t(first);
+            c->side_rows[1] = uint32_t(first + rows);
+            c->side_cols[0] = uint32_t(first + 2 * rows);
+            c->side_cols[1] = uint32_t(first + 2 * rows + cols);
+            side_bytes = (first + 2 * (rows + cols)) * c->elem - lat_bytes;
+        }
+    }
+    CREATE_TRY(hipMalloc(&c->lat[0], lat_bytes + std::max<size_t>(64, side_bytes)));
     if (c->two_lattice)
         CREATE_TRY(hipMalloc(&c->lat[1], lat_bytes + 64));
     CREATE_TRY(hipMalloc(&c->phase, c->N));
@@ -1287,6 +1607,7 @@ int lbm_init_populations(lbm_ctx *c)
     if (!c->phase_set)
         return fail(LBM_ERR_STATE, "lbm_init_populations: lbm_set_phase must come first");
     c->cur = 0;
+    c->side_valid = false;
     if (int rc = vtable(c).init(c))
         return rc;
     CodeParams cp{c->H,  c->V,  c->Hx, c->N, c->sub_h, c->sub_v, c->P, c->X0, c->buffered ? 1 : 0,
@@ -1569,6 +1890,7 @@ int lbm_set_state(lbm_ctx *c, const void *buf, size_t bytes)
     c->frame_iteration = ~0ull;
     c->cur = int(cur);
     c->snap_valid = false;
+    c->side_valid = false;
     return LBM_OK;
 }
 
@@ -1603,6 +1925,7 @@ int lbm_set_populations(lbm_ctx *c, const double *values, size_t count)
         return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_populations: count must be 9 * expanded node count");
     double *tmp = nullptr;
     LBM_HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
+    c->side_valid = false;
     hipError_t e = hipMemcpyAsync(tmp, values, count * sizeof(double), hipMemcpyHostToDevice, c->stream);
     int rc = e == hipSuccess ? vtable(c).encode(c, tmp)
                              : fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
@@ -1630,6 +1953,7 @@ int lbm_set_equilibrium_field(lbm_ctx *c, const double *rho, const double *jx, c
         return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_equilibrium_field: count must equal (H-2)*(V-2)");
     double *tmp = nullptr;
     LBM_HIP_TRY(hipMalloc(&tmp, 3 * count * sizeof(double)));
+    c->side_valid = false;
     hipError_t e = hipSuccess;
     const double *src[3] = {rho, jx, jy};
     for (int k = 0; k < 3 && e == hipSuccess; ++k)

@@ -40,6 +40,11 @@ RAGGED = [
     ("npol", 250, 70, 64), ("npol", 37, 21, 7), ("npol", 100, 30, 12), ("npol", 33, 18, 16),
     ("nsol", 250, 70, 64), ("nsol", 37, 21, 7), ("nsol", 100, 30, 12), ("nsol", 29, 16, 64),
     ("nsol", 20, 12, 6), ("npol", 2, 2, 4), ("nptl", 2, 2, 4),
+    # npol/nsol tiles one wave walks through: 32 x 32 (two rows per group),
+    # 30 x 31 (last group one row), 48 x 2 (one row per group); and tiles
+    # wider than a wave (65 x 2, 65 x 1), which the block-per-tile kernels take
+    ("npol", 100, 70, 1024), ("nsol", 150, 50, 1024), ("npol", 150, 40, 96),
+    ("npol", 300, 20, 130), ("nsol", 300, 20, 134),
 ]
 
 
