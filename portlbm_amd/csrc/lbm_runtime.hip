@@ -612,6 +612,22 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
         const uint32_t row = y0 + r;
         return (r < R && row < sv) ? uint32_t(p.code[place(x, y00 + row, P, p.X0)]) : 0u;
     };
+    // All codes of the lane's nodes, loaded once into its own LDS slots (one
+    // per group), so that a group's pulls wait for one round trip, not two.
+    extern __shared__ uint16_t sweep_codes[];
+    const uint32_t groups = (sv + R - 1u) / R;
+    uint16_t *codes = sweep_codes + (threadIdx.x / 64u) * groups * 64u + lane;
+    for (uint32_t g0 = 0; g0 < groups; g0 += 8u)
+    {
+        uint32_t v[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k)
+            v[k] = g0 + k < groups ? code_at((g0 + k) * R) : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k)
+            if (g0 + k < groups)
+                codes[(g0 + k) * 64u] = uint16_t(v[k]);
+    }
     // the pulls of the group starting at y0
     auto gather = [&](uint32_t y0, uint32_t code, C f[9]) {
         if (!(code & LBM_CODE_FLUID))
@@ -660,9 +676,9 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
     };
 
     C c6 = C(0), c7 = C(0), c8 = C(0);  // old 6, 7, 8 of the previous group's top row
-    for (uint32_t y0 = 0; y0 < sv; y0 += R)
+    for (uint32_t y0 = 0, g = 0; y0 < sv; y0 += R, ++g)
     {
-        const uint32_t code = code_at(y0);
+        const uint32_t code = codes[g * 64u];
         const C k6 = __shfl(c6, src6, 64), k7 = __shfl(c7, src7, 64), k8 = __shfl(c8, src8, 64);
         keep_top(y0, code, c6, c7, c8);
         C f[9] = {};
@@ -1105,8 +1121,10 @@ struct Impl
                 prime(c);
             const uint32_t tiles = c->cnt_h * c->cnt_v;
             const uint32_t per_block = TILE_SWEEP_BLOCK / 64u;
+            const uint32_t R = 64u / c->sub_h, groups = (c->sub_v + R - 1u) / R;
+            const size_t lds = size_t(per_block) * groups * 64u * sizeof(uint16_t);
             hipLaunchKernelGGL((k_step_tile_sweep<L, S>), dim3((tiles + per_block - 1u) / per_block),
-                               dim3(TILE_SWEEP_BLOCK), 0, c->stream, p, tiles);
+                               dim3(TILE_SWEEP_BLOCK), lds, c->stream, p, tiles);
             c->side_parity ^= 1;
         };
         switch (c->desc.algorithm)
