@@ -581,18 +581,29 @@ __device__ __forceinline__ bool row_segment_has(bool pred, uint32_t pn, int lane
     return (m & span) != 0ull;
 }
 
+// Nodes per lane and group: FP64 two (rows r and r + 64/sub_h), so that a
+// wave has twice the loads in flight -- its registers allow only 4 waves per
+// SIMD; the narrower storages run 5-6 waves with one node per lane.
+template <int S>
+constexpr uint32_t sweep_nodes_per_lane()
+{
+    return sizeof(typename Storage<S>::T) == 8 ? 2u : 1u;
+}
+
 template <int L, int S>
 __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams p, uint32_t tiles)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
     constexpr uint32_t E = L == LBM_LAYOUT_STREAM ? 1u : (L == LBM_LAYOUT_BUNDLE ? 3u : 9u);
+    constexpr uint32_t NL = sweep_nodes_per_lane<S>();
     T *lat = static_cast<T *>(p.dst);  // the side arrays follow the lattice in its allocation
     const uint32_t tile = blockIdx.x * (TILE_SWEEP_BLOCK / 64u) + threadIdx.x / 64u;
     if (tile >= tiles)
         return;  // whole waves
     const uint32_t sh = p.sub_h, sv = p.sub_v, P = p.P, Sd = p.S, Hx = p.Hx, Vx = p.Vx;
-    const uint32_t R = 64u / sh;  // node rows per group
+    const uint32_t R = 64u / sh;  // node rows per sub-group (one node per lane)
+    const uint32_t G = NL * R;    // node rows per group
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t r = lane / sh, i = lane - r * sh;
     const uint32_t by = tile / p.cnt_h, bx = tile - by * p.cnt_h;
@@ -607,28 +618,28 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
     const uint32_t top = (R - 1u) * sh;
     const int src6 = int(top + min(i + 1u, sh - 1u)), src7 = int(top + i), src8 = int(top + (i ? i - 1u : 0u));
 
-    // this lane's node code in the group starting at tile row y0 (0 outside the tile)
+    // this lane's node code in the sub-group starting at tile row y0 (0 outside the tile)
     auto code_at = [&](uint32_t y0) -> uint32_t {
         const uint32_t row = y0 + r;
         return (r < R && row < sv) ? uint32_t(p.code[place(x, y00 + row, P, p.X0)]) : 0u;
     };
     // All codes of the lane's nodes, loaded once into its own LDS slots (one
-    // per group), so that a group's pulls wait for one round trip, not two.
+    // per sub-group), so that a group's pulls wait for one round trip, not two.
     extern __shared__ uint16_t sweep_codes[];
-    const uint32_t groups = (sv + R - 1u) / R;
-    uint16_t *codes = sweep_codes + (threadIdx.x / 64u) * groups * 64u + lane;
-    for (uint32_t g0 = 0; g0 < groups; g0 += 8u)
+    const uint32_t groups = (sv + G - 1u) / G, slots = groups * NL;
+    uint16_t *codes = sweep_codes + (threadIdx.x / 64u) * slots * 64u + lane;
+    for (uint32_t s0 = 0; s0 < slots; s0 += 8u)
     {
         uint32_t v[8];
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k)
-            v[k] = g0 + k < groups ? code_at((g0 + k) * R) : 0u;
+            v[k] = s0 + k < slots ? code_at((s0 + k) * R) : 0u;
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k)
-            if (g0 + k < groups)
-                codes[(g0 + k) * 64u] = uint16_t(v[k]);
+            if (s0 + k < slots)
+                codes[(s0 + k) * 64u] = uint16_t(v[k]);
     }
-    // the pulls of the group starting at y0
+    // the pulls of the sub-group starting at y0
     auto gather = [&](uint32_t y0, uint32_t code, C f[9]) {
         if (!(code & LBM_CODE_FLUID))
             return;
@@ -664,7 +675,8 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
             f[d] = Storage<S>::load(lat, idx, bb ? 8 - d : d);
         }
     };
-    // own old populations 6, 7, 8 of the group's top row, when a group follows
+    // own old populations 6, 7, 8 of the group's top row (its last
+    // sub-group starts at y0), when a group follows
     auto keep_top = [&](uint32_t y0, uint32_t code, C &c6, C &c7, C &c8) {
         if (r == R - 1u && y0 + R < sv && code && !(code & LBM_CODE_SOLID))
         {
@@ -676,38 +688,54 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
     };
 
     C c6 = C(0), c7 = C(0), c8 = C(0);  // old 6, 7, 8 of the previous group's top row
-    for (uint32_t y0 = 0, g = 0; y0 < sv; y0 += R, ++g)
+    for (uint32_t y0 = 0, g = 0; y0 < sv; y0 += G, ++g)
     {
-        const uint32_t code = codes[g * 64u];
-        const C k6 = __shfl(c6, src6, 64), k7 = __shfl(c7, src7, 64), k8 = __shfl(c8, src8, 64);
-        keep_top(y0, code, c6, c7, c8);
-        C f[9] = {};
-        gather(y0, code, f);
-
-        const uint32_t row = y0 + r, y = y00 + row;
-        const uint32_t pn = place(x, y, P, p.X0);
-        const bool fluid = code & LBM_CODE_FLUID;
-        const bool fill = (code & LBM_CODE_SOLID) && row_segment_has<T, E>(fluid, pn, int(lane), lo, hi);
-        if (fluid)
-        {
-            if (y0 != 0u && r == 0u)
-            {
+        uint32_t code[NL];
 #pragma unroll
-                for (int d = 6; d < 9; ++d)
+        for (uint32_t h = 0; h < NL; ++h)
+            code[h] = codes[(g * NL + h) * 64u];
+        const C k6 = __shfl(c6, src6, 64), k7 = __shfl(c7, src7, 64), k8 = __shfl(c8, src8, 64);
+        keep_top(y0 + (NL - 1u) * R, code[NL - 1u], c6, c7, c8);
+        C f[NL][9] = {};
+#pragma unroll
+        for (uint32_t h = 0; h < NL; ++h)
+            gather(y0 + h * R, code[h], f[h]);
+
+        bool write[NL], fluid[NL];
+#pragma unroll
+        for (uint32_t h = 0; h < NL; ++h)
+        {
+            const uint32_t row = y0 + h * R + r, y = y00 + row;
+            const uint32_t pn = place(x, y, P, p.X0);
+            fluid[h] = code[h] & LBM_CODE_FLUID;
+            write[h] = fluid[h] || ((code[h] & LBM_CODE_SOLID) &&
+                                    row_segment_has<T, E>(fluid[h], pn, int(lane), lo, hi));
+            if (fluid[h])
+            {
+                if (h == 0u && y0 != 0u && r == 0u)
                 {
-                    const int sx = int(i) - cx_of(d);
-                    if (!((code >> d) & 1u) && sx >= 0 && sx < int(sh))
-                        f[d] = d == 6 ? k6 : (d == 7 ? k7 : k8);
+#pragma unroll
+                    for (int d = 6; d < 9; ++d)
+                    {
+                        const int sx = int(i) - cx_of(d);
+                        if (!((code[h] >> d) & 1u) && sx >= 0 && sx < int(sh))
+                            f[h][d] = d == 6 ? k6 : (d == 7 ? k7 : k8);
+                    }
                 }
+                finish_node<L, S>(p, code[h], xi, y - y / (sv + 1u), f[h]);
             }
-            finish_node<L, S>(p, code, xi, y - y / (sv + 1u), f);
         }
         // keep_top's loads have returned before any lane overwrites what they read
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        if (fluid || fill)
+#pragma unroll
+        for (uint32_t h = 0; h < NL; ++h)
         {
+            if (!write[h])
+                continue;
+            const uint32_t row = y0 + h * R + r, y = y00 + row;
+            const uint32_t pn = place(x, y, P, p.X0);
             T t[9];
-            node_bits<S>(f, pn, p.sr, t);
+            node_bits<S>(f[h], pn, p.sr, t);
 #pragma unroll
             for (int d = 0; d < 9; ++d)
             {
@@ -716,7 +744,7 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
                 else
                     lat[at<L>(pn, uint32_t(d), Sd)] = t[d];
             }
-            if (fluid)
+            if (fluid[h])
             {
                 if (buf_right)
                     for (uint32_t j = 0; j < 3; ++j)
@@ -1121,8 +1149,9 @@ struct Impl
                 prime(c);
             const uint32_t tiles = c->cnt_h * c->cnt_v;
             const uint32_t per_block = TILE_SWEEP_BLOCK / 64u;
-            const uint32_t R = 64u / c->sub_h, groups = (c->sub_v + R - 1u) / R;
-            const size_t lds = size_t(per_block) * groups * 64u * sizeof(uint16_t);
+            constexpr uint32_t NL = sweep_nodes_per_lane<S>();
+            const uint32_t R = 64u / c->sub_h, groups = (c->sub_v + NL * R - 1u) / (NL * R);
+            const size_t lds = size_t(per_block) * groups * NL * 64u * sizeof(uint16_t);
             hipLaunchKernelGGL((k_step_tile_sweep<L, S>), dim3((tiles + per_block - 1u) / per_block),
                                dim3(TILE_SWEEP_BLOCK), lds, c->stream, p, tiles);
             c->side_parity ^= 1;
