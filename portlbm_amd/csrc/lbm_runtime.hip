@@ -676,9 +676,12 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
         }
     };
     // own old populations 6, 7, 8 of the group's top row (its last
-    // sub-group starts at y0), when a group follows
+    // sub-group starts at y0), when a group follows.  Every node of the row
+    // but a solid one: ghost nodes (code 0) inside a tile -- the outlet ghost
+    // column when a buffer column separates it from nothing but the border
+    // node -- are pulled from too.
     auto keep_top = [&](uint32_t y0, uint32_t code, C &c6, C &c7, C &c8) {
-        if (r == R - 1u && y0 + R < sv && code && !(code & LBM_CODE_SOLID))
+        if (r == R - 1u && y0 + R < sv && !(code & LBM_CODE_SOLID))
         {
             const uint32_t pn = place(x, y00 + y0 + r, P, p.X0);
             c6 = Storage<S>::load(lat, at<L>(pn, 6u, Sd), 6);

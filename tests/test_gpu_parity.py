@@ -45,6 +45,10 @@ RAGGED = [
     # wider than a wave (65 x 2, 65 x 1), which the block-per-tile kernels take
     ("npol", 100, 70, 1024), ("nsol", 150, 50, 1024), ("npol", 150, 40, 96),
     ("npol", 300, 20, 130), ("nsol", 300, 20, 134),
+    # the buffered-outlet quirk with several row groups per tile: the outlet
+    # ghost column lies inside the last tile, and the border node's diagonal
+    # pull from it is handed down from the previous group's top row
+    ("npol", 97, 40, 256), ("nsol", 85, 40, 256), ("npol", 97, 70, 1024),
 ]
 
 
