@@ -352,6 +352,11 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
         }
         finish_node<L, S>(p, code, x0 + tx, y, f);
     }
+    // which places of each node row are fluid (a wave is a node row: B = 64)
+    __shared__ unsigned long long fluid_row[R];
+    const unsigned long long row_mask = __ballot(fluid);
+    if (tx == 0u)
+        fluid_row[ty] = row_mask;
     const bool odd = active && !fluid && !solid;
     if (__syncthreads_or(odd))
     {
@@ -384,20 +389,33 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
         }
     }
     __syncthreads();
-    // contiguous output arrays: one per node row (collision), three (bundle)
+    // contiguous output arrays: one per node row (collision), three (bundle).
+    // Each starts on a 64-byte segment (b0 is a multiple of 64 places); a
+    // segment that holds no fluid node is left unwritten -- solid places are
+    // never read -- and every other one is written whole.
+    constexpr uint32_t SE = 64u / sizeof(T);  // elements per segment
+    auto segment_fluid = [&](unsigned long long m, uint32_t e) {
+        const uint32_t s0 = e / SE * SE;
+        const uint32_t n0 = s0 / K, n1 = min((s0 + SE - 1u) / K, B - 1u);
+        const uint32_t n = n1 - n0 + 1u;
+        return (m >> n0) & (n >= 64u ? ~0ull : ((1ull << n) - 1ull));
+    };
     const uint32_t arrays = L == LBM_LAYOUT_COLLISION ? rows : 3u;
     const uint32_t per = K * len;                                  // elements per array
     for (uint32_t a = 0; a < arrays; ++a)
     {
         const uint32_t g0 = L == LBM_LAYOUT_COLLISION ? 9u * (b0 + a * p.P) : 3u * a * p.S + 3u * b0;
+        const unsigned long long m = fluid_row[L == LBM_LAYOUT_COLLISION ? a : 0u];
         const T *o = out + a * (K * B);
         const uint32_t nvec = per / VE;
         const V4 *lo = reinterpret_cast<const V4 *>(o);
         V4 *go = reinterpret_cast<V4 *>(dst + g0);
         for (uint32_t i = tid; i < nvec; i += NT)
-            __builtin_nontemporal_store(lo[i], go + i);
+            if (segment_fluid(m, i * VE))
+                __builtin_nontemporal_store(lo[i], go + i);
         for (uint32_t i = nvec * VE + tid; i < per; i += NT)
-            dst[g0 + i] = o[i];
+            if (segment_fluid(m, i))
+                dst[g0 + i] = o[i];
     }
 }
 
