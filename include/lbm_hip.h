@@ -214,10 +214,13 @@ int lbm_get_frame(lbm_ctx *ctx, double *density, double *x_velocity, double *y_v
 
 /* Current source lattice: 9 * expanded node count values in the context's
  * layout order (core/access.hpp:117-171), widened to double; shifted
- * storage is returned as f_i (the weight added back).  npol/nsol with tiles
- * at most 64 nodes wide keep the buffer copies of a step outside the lattice,
- * so the lattice's buffer rows/columns hold what they held when last
- * refreshed (a split step refreshes them); every other node is current. */
+ * storage is returned as f_i (the weight added back).  Fluid and ghost nodes
+ * are current.  Solid nodes' populations are never read (half-way
+ * bounce-back takes the fluid node's own), and the fused kernels leave them
+ * zero or as they were.  npol/nsol with tiles at most 64 nodes wide keep
+ * the buffer copies of a step outside the lattice, so the lattice's buffer
+ * rows/columns hold what they held when last refreshed (a split step
+ * refreshes them). */
 int lbm_get_populations(lbm_ctx *ctx, double *values, size_t count);
 int lbm_set_populations(lbm_ctx *ctx, const double *values, size_t count);
 
