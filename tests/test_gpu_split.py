@@ -28,7 +28,8 @@ def _doc(alg, layout, storage="f64", rounding="nearest", scenario="circle"):
     ("lptl", "stream", "bf16-shifted", "nearest"), ("nptl", "collision", "f64", "nearest"),
     ("nptl", "collision", "f16-shifted", "stochastic"), ("lptl", "collision", "bf16-shifted", "nearest"),
     ("nptl", "bundle", "f32", "nearest"), ("lptl", "bundle", "f16-shifted", "nearest"),
-    ("nptl", "stream", "f16-shifted", "stochastic")])
+    ("nptl", "stream", "f16-shifted", "stochastic"), ("npol", "bundle", "f32", "nearest"),
+    ("nsol", "stream", "f32", "nearest")])
 def test_split_steps_equal_fused_steps(alg, layout, storage, rounding):
     doc = _doc(alg, layout, storage, rounding)
     with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
