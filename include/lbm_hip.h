@@ -23,7 +23,7 @@
  *     (EmplaceBounceBackKernel, OutletUpdateKernel, StreamCollideKernel and
  *      the buffer-copy kernels of every algorithm are fused into one launch
  *      per step; the one-lattice schemes with tiles wider than 64 nodes, and
- *      their FP64 collision layout, add one buffer refresh)
+ *      npol's FP64 collision layout, add the buffer refresh: two launches)
  *   SYCLAlgorithm::copy_macroscopic_observables_to_cpu   lbm_get_macros
  *     include/lbm/execution/sycl_algorithm.hpp:90-104
  *   the same copies enqueued after each frame step       lbm_run_frames /

@@ -1597,11 +1597,12 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     // npol/nsol tiles at most one wave wide: the tile sweep's side arrays go
     // after the lattice in the same allocation (32-bit element offsets), two
     // of each kind, each starting on 256 bytes
-    // (FP64 in the collision layout excepted: its strided 72-byte nodes ran
-    // faster with the block-per-tile kernels and the refresh launches)
+    // (npol's FP64 collision layout excepted: its strided 72-byte nodes ran
+    // faster with the block-per-tile kernel and the refresh launches, 15.5
+    // against 14.6 GFLUPS on the porous channel; nsol's ran 14.8 against 14.0)
     size_t side_bytes = 0;
     if (c->buffered && c->sub_h <= TILE_SWEEP_MAX_WIDTH &&
-        !(desc->layout == LBM_LAYOUT_COLLISION && c->elem == 8))
+        !(desc->algorithm == LBM_ALG_NPOL && desc->layout == LBM_LAYOUT_COLLISION && c->elem == 8))
     {
         const size_t align = 256 / c->elem;
         auto round_up = [&](size_t n) { return (n + align - 1) / align * align; };
