@@ -549,7 +549,8 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
 }
 
 // npol and nsol with tiles at most one wave wide: one wave per tile, walking
-// up the tile R = 64 / sub_h node rows at a time, with no barrier and no
+// up the tile a group of rows at a time (64 / sub_h rows, one node per lane;
+// twice that in FP64/FP32, two nodes per lane), with no barrier and no
 // buffer-refresh launches.
 //
 // The buffer rows and columns of the lattice carry copies of tile-edge
