@@ -1497,8 +1497,11 @@ void lbm_destroy(lbm_ctx *c)
 {
     if (!c)
         return;
+    // both streams drained before anything they may still read or write is freed
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream)
+        (void)hipStreamSynchronize(c->copy_stream);
     (void)hipFree(c->lat[0]);
     (void)hipFree(c->lat[1]);
     (void)hipFree(c->phase);
