@@ -67,6 +67,28 @@ def test_staged_kernels_equal_split_steps_on_ragged_shapes(alg, layout, storage,
             np.testing.assert_array_equal(a, b)
 
 
+# npol/nsol tiles with several row groups, so that the tile sweep's hand-down
+# between groups, its side arrays and (FP32) two nodes per lane are checked in
+# the reduced-precision storages too: 32 x 32 tiles (FP32: groups of 4 rows),
+# 30 x 31 (last group partial), 16 x 16 with the buffered-outlet quirk.
+@pytest.mark.parametrize("alg,w,h,wg,layout,storage,rounding", [
+    ("npol", 100, 70, 1024, "stream", "f32", "nearest"),
+    ("nsol", 150, 50, 1024, "bundle", "f32", "nearest"),
+    ("nsol", 150, 50, 1024, "stream", "f16-shifted", "stochastic"),
+    ("npol", 97, 40, 256, "collision", "bf16-shifted", "nearest"),
+    ("npol", 97, 40, 256, "stream", "f16-shifted", "nearest")])
+def test_tile_sweep_equals_split_steps_on_multi_group_tiles(alg, w, h, wg, layout, storage, rounding):
+    doc = pa.settings(alg, layout, inner_h=w, inner_v=h, scenario="circle", work_group_size=wg, time_steps=1,
+                      tau=0.7, rho_in=1.05, storage=storage, rounding=rounding)
+    with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
+        keep = _fluid_slots(fused)
+        fused.run(41)
+        split.run_split(41)
+        np.testing.assert_array_equal(split.populations()[keep], fused.populations()[keep])
+        for a, b in zip(split.macros(), fused.macros()):
+            np.testing.assert_array_equal(a, b)
+
+
 def _fluid_slots(lat):
     """Population indices (layout order of lbm_get_populations) of the fluid nodes."""
     N = lat.N
