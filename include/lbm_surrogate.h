@@ -7,7 +7,7 @@
  * No counterpart in the reference (its solver always starts from rest,
  * include/lbm/core/domain_initialization.hpp:311-403); callers use the
  * prediction only to warm-start a lattice from its equilibrium, and keep the
- * plain solve as the judge and fallback.
+ * plain solve as the arbiter of convergence and as the fallback.
  *
  * Network (NCHW float32; every conv 3x3, zero padding, ReLU except the last;
  * pool = 2x2 mean; upadd = nearest 2x upsampling plus skip):

@@ -2,7 +2,7 @@
 (portlbm_amd/data/surrogate_porous_256.npz, trained on the GPU by this
 package's kernels on 18 694 solver-generated steady states) against the
 plain at-rest start on held-out 256 x 256 porous channels, FP64 lattice,
-residual ||j^{t+100} - j^t|| / ||j^{t+100}|| < 1e-6.  The plain solve judges
+residual ||j^{t+100} - j^t|| / ||j^{t+100}|| < 1e-6.  The plain solve decides
 convergence; a warm start whose residual fails to fall is reset to rest."""
 import numpy as np
 import pytest
