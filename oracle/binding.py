@@ -16,7 +16,10 @@ LAYOUTS = {"collision": 0, "stream": 1, "bundle": 2}
 SCENARIOS = {"Hagen-Poiseuille": 0, "walls": 1, "circle": 2, "square": 3, "wing": 4, "skyscraper": 5,
              "porous": 6, "plate": 7, "porous-circles": 8, "cylinder": 9}
 
-_lib: ctypes.CDLL | None = None
+# the instances of oracle.c: real_type double (the reference's default build)
+# and float (its USE_FLOAT build)
+LIBRARIES = {"f64": _HERE / "liboracle.so", "f32": _HERE / "liboracle_f32.so"}
+_libs: dict[str, ctypes.CDLL] = {}
 
 
 class ScenarioParams(ctypes.Structure):
@@ -25,13 +28,15 @@ class ScenarioParams(ctypes.Structure):
                 ("cylinder_radius", ctypes.c_double), ("max_attempts", ctypes.c_uint64)]
 
 
-def library() -> ctypes.CDLL:
-    global _lib
-    if _lib is None:
-        path = _HERE / "liboracle.so"
+def library(real: str = "f64", path: str | Path | None = None) -> ctypes.CDLL:
+    """The oracle instance of `real` ("f64" or "f32"), or the build at `path`
+    (a test's own build of oracle.c, e.g. with a planted mistake)."""
+    path = Path(path) if path is not None else LIBRARIES[real]
+    key = str(path)
+    if key not in _libs:
         if not path.exists():
             raise RuntimeError(f"{path} missing: run `make -C oracle` (or portlbm_amd.build)")
-        lib = ctypes.CDLL(str(path))
+        lib = ctypes.CDLL(key)
         P, U, D = ctypes.c_void_p, ctypes.c_uint, ctypes.c_double
         lib.oracle_domain.argtypes = [ctypes.c_int, U, U, U, P]
         lib.oracle_domain.restype = ctypes.c_int
@@ -60,8 +65,9 @@ def library() -> ctypes.CDLL:
         lib.oracle_reduce.argtypes = [P, P]
         lib.oracle_set_threads.argtypes = [ctypes.c_int]
         lib.oracle_max_threads.restype = ctypes.c_int
-        _lib = lib
-    return _lib
+        lib.oracle_real_bytes.restype = ctypes.c_size_t
+        _libs[key] = lib
+    return _libs[key]
 
 
 def _p(a: np.ndarray) -> ctypes.c_void_p:
@@ -90,10 +96,14 @@ def max_threads() -> int:
 
 
 class Simulation:
-    """The reference's simulation of one settings document, on the CPU."""
+    """The reference's simulation of one settings document, on the CPU.
 
-    def __init__(self, doc: dict):
-        lib = library()
+    real: "f64" (real_type double) or "f32" (the USE_FLOAT build); lib_path
+    overrides the library (tests that plant a mistake in their own build)."""
+
+    def __init__(self, doc: dict, real: str = "f64", lib_path: str | Path | None = None):
+        self._lib = lib = library(real, lib_path)
+        self.dtype = np.float64 if lib.oracle_real_bytes() == 8 else np.float32
         alg, dom, phy = doc["algorithmic"], doc["domain"], doc["physical"]
         ext = doc.get("mi355x", {})
         H, V = dom["horizontalNodes"] + 2, dom["verticalNodes"] + 2
@@ -120,7 +130,7 @@ class Simulation:
 
     def close(self) -> None:
         if self._s:
-            library().oracle_destroy(self._s)
+            self._lib.oracle_destroy(self._s)
             self._s = None
 
     def __del__(self):
@@ -130,39 +140,42 @@ class Simulation:
             pass
 
     def phase(self) -> np.ndarray:
-        ptr = library().oracle_phase(self._s)
+        ptr = self._lib.oracle_phase(self._s)
         return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int8)), (self.N,)).copy()
 
     def set_phase(self, phase: np.ndarray) -> None:
         phase = np.ascontiguousarray(phase, dtype=np.int8)
-        if library().oracle_set_phase(self._s, _p(phase), phase.size) != 0:
+        if self._lib.oracle_set_phase(self._s, _p(phase), phase.size) != 0:
             raise ValueError("phase size mismatch")
 
     def init(self) -> None:
-        library().oracle_init(self._s)
+        self._lib.oracle_init(self._s)
 
     def run(self, steps: int) -> None:
-        library().oracle_run(self._s, steps)
+        self._lib.oracle_run(self._s, steps)
 
     def populations(self) -> np.ndarray:
-        ptr = library().oracle_populations(self._s)
-        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)), (9 * self.N,)).copy()
+        """The source lattice in layout order, as real_type (float64 or float32)."""
+        ptr = self._lib.oracle_populations(self._s)
+        ctype = ctypes.c_double if self.dtype == np.float64 else ctypes.c_float
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (9 * self.N,)).copy()
 
     def set_populations(self, values: np.ndarray) -> None:
-        values = np.ascontiguousarray(values, dtype=np.float64)
-        library().oracle_set_populations(self._s, _p(values), values.size)
+        """Both lattices := values (cast to real_type)."""
+        values = np.ascontiguousarray(values, dtype=self.dtype)
+        self._lib.oracle_set_populations(self._s, _p(values), values.size)
 
     def results(self) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
         out = [np.empty(self.D) for _ in range(4)]
-        library().oracle_results(self._s, *map(_p, out))
+        self._lib.oracle_results(self._s, *map(_p, out))
         return tuple(out)  # type: ignore[return-value]
 
     def node_codes(self) -> np.ndarray:
         out = np.empty(self.N, dtype=np.uint16)
-        library().oracle_node_codes(self._s, _p(out))
+        self._lib.oracle_node_codes(self._s, _p(out))
         return out
 
     def reduce(self) -> dict:
         out = np.zeros(6)
-        library().oracle_reduce(self._s, _p(out))
+        self._lib.oracle_reduce(self._s, _p(out))
         return {"mass": out[0], "jx": out[1], "jy": out[2], "fluid": int(out[3]), "fx": out[4], "fy": out[5]}

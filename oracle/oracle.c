@@ -2,14 +2,22 @@
  * oracle.c -- CPU restatement of SC-SGS/PortLBM's D2Q9 BGK time step.
  *
  * TEST INFRASTRUCTURE ONLY (see oracle.h).  Built with -ffp-contract=off so
- * that every floating-point operation is the IEEE double operation the
- * reference source text writes down, in the same order.
+ * that every floating-point operation is the IEEE operation the reference
+ * source text writes down, in the same order.
+ *
+ * `real` is the reference's real_type (constants.hpp:17-21): double by
+ * default (liboracle.so), float when built with -DOR_REAL=float
+ * (liboracle_f32.so, the reference's USE_FLOAT build).  Expressions keep the
+ * reference's literals, so C's usual arithmetic conversions promote exactly
+ * what C++'s promote there: in the float build the terms 9.0 / 2 * ... and
+ * 3.0 / 2 * ... of the equilibrium and the outlet's rebuilt populations are
+ * evaluated in double and rounded to float on assignment, the rest in float.
  *
  * Reference files restated here (paths relative to the reference root):
- *   include/lbm/core/constants.hpp:31-39        lattice weights
- *   include/lbm/core/access.hpp:117-171         layouts
- *   include/lbm/core/access.hpp:203-290         neighbour / node / result indices
- *   include/lbm/core/access.hpp:321-362         (buffered) node access
+ *   include/lbm/core/constants.hpp:17-21,38-39  real_type, lattice weights
+ *   include/lbm/core/access.hpp:41-110          layouts (Collision/Stream/BundleAccessor::at)
+ *   include/lbm/core/access.hpp:142-229         neighbour / node / result indices
+ *   include/lbm/core/access.hpp:244-302         (buffered) node access
  *   include/lbm/core/simulation.hpp:333-350     host equilibrium
  *   include/lbm/core/simulation.hpp:380-444     power functions
  *   src/lbm/core/simulation.cpp:223-382         domain extents
@@ -30,20 +38,37 @@
 #include <omp.h>
 #endif
 
-/* ---- constants.hpp:38-39 ---- */
-static const double W[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 9, 1.0 / 36};
+#ifndef OR_REAL
+#define OR_REAL double
+#endif
+typedef OR_REAL real;
 
-/* ---- access.hpp:117-171 ---- */
+size_t oracle_real_bytes(void) { return sizeof(real); }
+
+/* OR_PLANT selects one deliberate mistake in the collision, for the tests
+ * that show the oracle's checks catch it (never defined in a shipped build):
+ *   1  relaxation factor tau instead of 1 / tau
+ *   2  coefficient 3.0 instead of 9.0 / 2 on (c.j)^2
+ *   3  coefficient 1.0 instead of 3.0 / 2 on |j|^2
+ *   4  term 3 * c.j dropped */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
+/* ---- constants.hpp:38-39 (weights are real_type) ---- */
+static const real W[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 9, 1.0 / 36};
+
+/* ---- access.hpp:41-110 ---- */
 static inline unsigned at(int layout, unsigned node, unsigned d, unsigned total)
 {
     if (layout == OR_COLLISION)
-        return 9u * node + d;                              /* access.hpp:117-121 */
+        return 9u * node + d;                              /* access.hpp:56-60 */
     if (layout == OR_STREAM)
-        return total * d + node;                           /* access.hpp:142-146 */
-    return 3u * (d / 3u) * total + (d % 3u) + 3u * node;    /* access.hpp:167-171 */
+        return total * d + node;                           /* access.hpp:81-85 */
+    return 3u * (d / 3u) * total + (d % 3u) + 3u * node;    /* access.hpp:106-110 */
 }
 
-/* ---- access.hpp:203-213 ---- */
+/* ---- access.hpp:142-152 ---- */
 static inline unsigned get_neighbor(unsigned node, unsigned d, unsigned h)
 {
     int y_offset = (int)(d / 3) - 1;
@@ -51,14 +76,14 @@ static inline unsigned get_neighbor(unsigned node, unsigned d, unsigned h)
     return node + y_offset * h + x_offset;
 }
 
-/* ---- access.hpp:269-273 and 286-290 ---- */
+/* ---- access.hpp:208-212 and 225-229 ---- */
 static inline unsigned result_index_xy(unsigned x, unsigned y, unsigned h) { return (x - 1) + (y - 1) * (h - 2); }
 static inline unsigned result_index_linear(unsigned node, unsigned h)
 {
     return (((node - h) / h) * (h - 2) + (node - 1) % h);
 }
 
-/* ---- access.hpp:321-329 / 353-362 ---- */
+/* ---- access.hpp:260-268 / 292-301 ---- */
 static inline unsigned node_index(int buffered, unsigned x, unsigned y, unsigned sub_v, unsigned sub_h, unsigned hx)
 {
     if (!buffered)
@@ -67,7 +92,7 @@ static inline unsigned node_index(int buffered, unsigned x, unsigned y, unsigned
 }
 
 /* ---- simulation.hpp:333-350 ---- */
-static void maxwell_boltzmann(double ux, double uy, double rho, double *target)
+static void maxwell_boltzmann(real ux, real uy, real rho, real *target)
 {
     int cx = 0, cy = 0;
     for (int d = 0; d < 9; ++d)
@@ -241,10 +266,10 @@ struct oracle_sim
 {
     int alg, layout, buffered, nan_protection;
     unsigned H, V, Hx, Vx, N, sub_h, sub_v, cnt_h, cnt_v;
-    double tau, rho_in, ux_in, uy_in, rho_out, ux_out, uy_out;
+    real tau, rho_in, ux_in, uy_in, rho_out, ux_out, uy_out; /* Properties, real_type */
     int8_t *phase;
-    double *lat0, *lat1; /* lat0 is the source lattice (data->distribution_values_0) */
-    double *rho, *jx, *jy, *jabs;
+    real *lat0, *lat1; /* lat0 is the source lattice (data->distribution_values_0) */
+    real *rho, *jx, *jy, *jabs;
     unsigned D;
     unsigned long iteration;
 };
@@ -281,23 +306,24 @@ oracle_sim *oracle_create(int alg, int layout, unsigned H, unsigned V, unsigned 
     s->cnt_h = e[4];
     s->cnt_v = e[5];
     s->N = e[6];
-    s->tau = tau;
-    s->rho_in = rho_in;
-    s->ux_in = ux_in;
-    s->uy_in = uy_in;
-    s->rho_out = rho_out;
-    s->ux_out = ux_out;
-    s->uy_out = uy_out;
+    /* json_to_properties reads these as real_type (file_interaction.cpp:58-64) */
+    s->tau = (real)tau;
+    s->rho_in = (real)rho_in;
+    s->ux_in = (real)ux_in;
+    s->uy_in = (real)uy_in;
+    s->rho_out = (real)rho_out;
+    s->ux_out = (real)ux_out;
+    s->uy_out = (real)uy_out;
     s->phase = (int8_t *)malloc(s->N);
     memset(s->phase, 0xff, s->N); /* simulation.cpp:424 */
-    s->lat0 = (double *)calloc(9ull * s->N, sizeof(double));
+    s->lat0 = (real *)calloc(9ull * s->N, sizeof(real));
     if (!s->buffered)
-        s->lat1 = (double *)calloc(9ull * s->N, sizeof(double));
+        s->lat1 = (real *)calloc(9ull * s->N, sizeof(real));
     s->D = (H - 2) * (V - 2);
-    s->rho = (double *)malloc(sizeof(double) * (s->D ? s->D : 1));
-    s->jx = (double *)calloc(s->D ? s->D : 1, sizeof(double));
-    s->jy = (double *)calloc(s->D ? s->D : 1, sizeof(double));
-    s->jabs = (double *)calloc(s->D ? s->D : 1, sizeof(double));
+    s->rho = (real *)malloc(sizeof(real) * (s->D ? s->D : 1));
+    s->jx = (real *)calloc(s->D ? s->D : 1, sizeof(real));
+    s->jy = (real *)calloc(s->D ? s->D : 1, sizeof(real));
+    s->jabs = (real *)calloc(s->D ? s->D : 1, sizeof(real));
     for (unsigned i = 0; i < s->D; ++i)
         s->rho[i] = -1.0; /* simulation.cpp:407 */
     return s;
@@ -320,7 +346,7 @@ void oracle_destroy(oracle_sim *s)
 size_t oracle_node_count(const oracle_sim *s) { return s->N; }
 size_t oracle_result_count(const oracle_sim *s) { return s->D; }
 const int8_t *oracle_phase(const oracle_sim *s) { return s->phase; }
-const double *oracle_populations(const oracle_sim *s) { return s->lat0; }
+const void *oracle_populations(const oracle_sim *s) { return s->lat0; }
 void oracle_extents(const oracle_sim *s, unsigned out[7])
 {
     out[0] = s->Hx;
@@ -332,13 +358,13 @@ void oracle_extents(const oracle_sim *s, unsigned out[7])
     out[6] = s->N;
 }
 
-void oracle_set_populations(oracle_sim *s, const double *v, size_t count)
+void oracle_set_populations(oracle_sim *s, const void *v, size_t count)
 {
     if (count != 9ull * s->N)
         return;
-    memcpy(s->lat0, v, count * sizeof(double));
+    memcpy(s->lat0, v, count * sizeof(real));
     if (s->lat1)
-        memcpy(s->lat1, v, count * sizeof(double));
+        memcpy(s->lat1, v, count * sizeof(real));
 }
 
 int oracle_set_phase(oracle_sim *s, const int8_t *phase, size_t n)
@@ -349,16 +375,19 @@ int oracle_set_phase(oracle_sim *s, const int8_t *phase, size_t n)
     return 0;
 }
 
+static void widen(double *out, const real *in, unsigned n)
+{
+    if (out)
+        for (unsigned i = 0; i < n; ++i)
+            out[i] = (double)in[i];
+}
+
 void oracle_results(const oracle_sim *s, double *rho, double *jx, double *jy, double *jabs)
 {
-    if (rho)
-        memcpy(rho, s->rho, s->D * sizeof(double));
-    if (jx)
-        memcpy(jx, s->jx, s->D * sizeof(double));
-    if (jy)
-        memcpy(jy, s->jy, s->D * sizeof(double));
-    if (jabs)
-        memcpy(jabs, s->jabs, s->D * sizeof(double));
+    widen(rho, s->rho, s->D);
+    widen(jx, s->jx, s->D);
+    widen(jy, s->jy, s->D);
+    widen(jabs, s->jabs, s->D);
 }
 
 /* ---- scenarios: domain_initialization.hpp:41-192 (unsigned semantics) ---- */
@@ -565,7 +594,7 @@ int oracle_build_phase(oracle_sim *s, int scenario, const oracle_scenario_params
 /* ---- initial populations: domain_initialization.hpp:311-403, 460-467 ---- */
 void oracle_init(oracle_sim *s)
 {
-    double rest[9], values[18];
+    real rest[9], values[18];
     maxwell_boltzmann(0, 0, 1, rest);
     for (unsigned n = 0; n < s->N; ++n)
         for (int d = 0; d < 9; ++d)
@@ -583,7 +612,7 @@ void oracle_init(oracle_sim *s)
                 s->lat0[at(s->layout, node, (unsigned)i, s->N)] = values[i + 9 * id0];
         }
     if (s->lat1)
-        memcpy(s->lat1, s->lat0, 9ull * s->N * sizeof(double));
+        memcpy(s->lat1, s->lat0, 9ull * s->N * sizeof(real));
     s->iteration = 0;
     for (unsigned i = 0; i < s->D; ++i)
     {
@@ -593,20 +622,24 @@ void oracle_init(oracle_sim *s)
 }
 
 /* ---- the collision of one node: nptl_kernels.hpp:381-437 ---- */
-static inline double clamp_out(double v, int on)
+static inline real clamp_out(real v, int on)
 {
     if (on && (isnan(v) || v > FLT_MAX))
         return 0;
     return v;
 }
 
-static inline void collide_and_store(oracle_sim *s, double *dst, unsigned n, unsigned r, const double *f_in)
+static inline void collide_and_store(oracle_sim *s, real *dst, unsigned n, unsigned r, const real *f_in)
 {
-    double f[9];
+    real f[9];
     memcpy(f, f_in, sizeof(f));
-    double density = 0, fx = 0, fy = 0, absolute = 0, result = 0, result_buf = 0;
+    real density = 0, fx = 0, fy = 0, absolute = 0, result = 0, result_buf = 0;
     int cx = 0, cy = 0;
-    const double omega = 1 / s->tau; /* relaxation_time_inverse, nptl_kernels.hpp:341 */
+#if OR_PLANT == 1
+    const real omega = s->tau;
+#else
+    const real omega = 1 / s->tau; /* relaxation_time_inverse, nptl_kernels.hpp:341 */
+#endif
     for (int d = 0; d < 9; ++d)
     {
         density += f[d];
@@ -621,11 +654,19 @@ static inline void collide_and_store(oracle_sim *s, double *dst, unsigned n, uns
         cx = (d % 3) - 1;
         cy = (d / 3) - 1;
         result_buf = cx * fx + cy * fy;
+#if OR_PLANT == 2
+        result = W[d] * (density + 3 * result_buf + 3.0 * result_buf * result_buf - 3.0 / 2 * absolute);
+#elif OR_PLANT == 3
+        result = W[d] * (density + 3 * result_buf + 9.0 / 2 * result_buf * result_buf - 1.0 * absolute);
+#elif OR_PLANT == 4
+        result = W[d] * (density + 9.0 / 2 * result_buf * result_buf - 3.0 / 2 * absolute);
+#else
         result = W[d] * (density + 3 * result_buf + 9.0 / 2 * result_buf * result_buf - 3.0 / 2 * absolute);
+#endif
         result = -omega * (f[d] - result) + f[d];
         dst[at(s->layout, n, (unsigned)d, s->N)] = result;
     }
-    absolute = sqrt(absolute);
+    absolute = (real)sqrt(absolute); /* sycl::sqrt in real_type: sqrt rounded once more is exact for float */
     s->rho[r] = clamp_out(density, s->nan_protection);
     s->jx[r] = clamp_out(fx, s->nan_protection);
     s->jy[r] = clamp_out(fy, s->nan_protection);
@@ -634,7 +675,7 @@ static inline void collide_and_store(oracle_sim *s, double *dst, unsigned n, uns
 
 /* ---- EmplaceBounceBackKernel (nptl_kernels.hpp:471-491, lptl_kernels.hpp:451-467,
  *      npol_kernels.hpp:232-253, nsol_kernels.hpp:81-102) ---- */
-static inline void emplace_one(oracle_sim *s, double *lat, unsigned n, unsigned h)
+static inline void emplace_one(oracle_sim *s, real *lat, unsigned n, unsigned h)
 {
     if (s->phase[n] != 1)
         return;
@@ -648,7 +689,7 @@ static inline void emplace_one(oracle_sim *s, double *lat, unsigned n, unsigned 
 
 static void bounce_back(oracle_sim *s)
 {
-    double *lat = s->lat0;
+    real *lat = s->lat0;
     if (s->alg == OR_LPTL)
     {
         for (unsigned id = 0; id < s->H * s->V; ++id)
@@ -671,7 +712,7 @@ static void bounce_back(oracle_sim *s)
 /* ---- buffered.hpp:74-93 and 133-152 ---- */
 static void refresh_buffers(oracle_sim *s)
 {
-    double *lat = s->lat0;
+    real *lat = s->lat0;
     const unsigned Hx = s->Hx, N = s->N;
     if (s->cnt_v > 1)
         for (unsigned i0 = 0; i0 < s->cnt_v - 1; ++i0)
@@ -702,7 +743,7 @@ static void refresh_buffers(oracle_sim *s)
 /* ---- OutletUpdateKernel: non_buffered.hpp:75-130, buffered.hpp:268-326 ---- */
 static void outlet(oracle_sim *s)
 {
-    double *src = s->lat0;
+    real *src = s->lat0;
     const unsigned Hx = s->Hx, N = s->N, H = s->H;
     const unsigned missing[3] = {0, 3, 6};
     for (unsigned id = 0; id + 4 < s->V; ++id)
@@ -716,14 +757,14 @@ static void outlet(oracle_sim *s)
         }
         else
             border = (H - 2) + (unsigned)y * Hx;
-        double f_inverse[3];
-        const double f_1 = src[at(s->layout, get_neighbor(border, 7, Hx), 1, N)];
-        const double f_4 = src[at(s->layout, border, 4, N)];
-        const double f_7 = src[at(s->layout, get_neighbor(border, 1, Hx), 7, N)];
+        real f_inverse[3];
+        const real f_1 = src[at(s->layout, get_neighbor(border, 7, Hx), 1, N)];
+        const real f_4 = src[at(s->layout, border, 4, N)];
+        const real f_7 = src[at(s->layout, get_neighbor(border, 1, Hx), 7, N)];
         for (int i = 0; i < 3; ++i)
             f_inverse[i] = src[at(s->layout, get_neighbor(border, missing[i], Hx), 8 - missing[i], N)];
-        const double x_velocity = -s->rho_out + (f_1 + f_4 + f_7 + 2 * (f_inverse[0] + f_inverse[1] + f_inverse[2]));
-        const double y_velocity = s->jy[result_index_xy(H - 3, (unsigned)y, H)];
+        const real x_velocity = -s->rho_out + (f_1 + f_4 + f_7 + 2 * (f_inverse[0] + f_inverse[1] + f_inverse[2]));
+        const real y_velocity = s->jy[result_index_xy(H - 3, (unsigned)y, H)];
         src[at(s->layout, get_neighbor(border, 8 - missing[0], Hx), missing[0], N)] =
             f_inverse[0] - 0.5 * (f_1 - f_7) - 1.0 / 6 * x_velocity - 0.5 * y_velocity;
         src[at(s->layout, get_neighbor(border, 8 - missing[1], Hx), missing[1], N)] =
@@ -735,7 +776,7 @@ static void outlet(oracle_sim *s)
 
 /* ---- StreamCollideKernel, two-lattice pull: nptl_kernels.hpp:350-439,
  *      lptl_kernels.hpp:331-414 ---- */
-static inline void pull9(const oracle_sim *s, const double *src, unsigned n, double *f)
+static inline void pull9(const oracle_sim *s, const real *src, unsigned n, real *f)
 {
     for (int d = 0; d < 9; ++d)
         f[d] = src[at(s->layout, get_neighbor(n, 8 - (unsigned)d, s->Hx), (unsigned)d, s->N)];
@@ -743,8 +784,8 @@ static inline void pull9(const oracle_sim *s, const double *src, unsigned n, dou
 
 static void stream_collide_two_lattice(oracle_sim *s)
 {
-    const double *src = s->lat0;
-    double *dst = s->lat1;
+    const real *src = s->lat0;
+    real *dst = s->lat1;
     if (s->alg == OR_NPTL)
     {
 #pragma omp parallel for schedule(static) num_threads(oracle_max_threads())
@@ -754,7 +795,7 @@ static void stream_collide_two_lattice(oracle_sim *s)
                 const unsigned x = gx + 1, y = gy + 1, n = x + y * s->Hx;
                 if (!s->phase[n])
                 {
-                    double f[9];
+                    real f[9];
                     pull9(s, src, n, f);
                     collide_and_store(s, dst, n, result_index_xy(x, y, s->H), f);
                 }
@@ -766,7 +807,7 @@ static void stream_collide_two_lattice(oracle_sim *s)
         for (unsigned id = 0; id < s->H * s->V; ++id)
             if (!s->phase[id])
             {
-                double f[9];
+                real f[9];
                 pull9(s, src, id, f);
                 collide_and_store(s, dst, id, result_index_linear(id, s->H), f);
             }
@@ -783,7 +824,7 @@ static void stream_collide_npol(oracle_sim *s)
     for (unsigned wg = 0; wg < s->cnt_h * s->cnt_v; ++wg)
     {
         const unsigned bx = wg % s->cnt_h, by = wg / s->cnt_h;
-        double *priv = (double *)malloc(sizeof(double) * 9 * sh * sv);
+        real *priv = (real *)malloc(sizeof(real) * 9 * sh * sv);
         for (unsigned ly = 0; ly < sv; ++ly)
             for (unsigned lx = 0; lx < sh; ++lx)
             {
@@ -812,7 +853,7 @@ static void stream_collide_npol(oracle_sim *s)
  *      group by work-group: the three barrier-separated phases run for all
  *      items of the group before the next phase starts.  `local` holds two
  *      values per LDS node with pitch sub_h + 4 (:161-162). ---- */
-static inline double read_guarded(const oracle_sim *s, unsigned node, unsigned d)
+static inline real read_guarded(const oracle_sim *s, unsigned node, unsigned d)
 {
     /* the reference's reads past the last node (collision layout, top-right
      * item) feed only non-fluid items, whose values are never used */
@@ -830,8 +871,8 @@ static void stream_collide_nsol(oracle_sim *s)
     for (unsigned wg = 0; wg < s->cnt_h * s->cnt_v; ++wg)
     {
         const unsigned bx = wg % s->cnt_h, by = wg / s->cnt_h;
-        double *local = (double *)calloc(2ull * pitch * (sv + 2), sizeof(double));
-        double *priv = (double *)calloc(9ull * items, sizeof(double));
+        real *local = (real *)calloc(2ull * pitch * (sv + 2), sizeof(real));
+        real *priv = (real *)calloc(9ull * items, sizeof(real));
         unsigned *node = (unsigned *)malloc(sizeof(unsigned) * items);
         unsigned *own = (unsigned *)malloc(sizeof(unsigned) * items);
         for (unsigned ly = 0; ly < lh; ++ly)
@@ -848,7 +889,7 @@ static void stream_collide_nsol(oracle_sim *s)
         for (unsigned i = 0; i < items; ++i)
         {
             const unsigned n = node[i];
-            double *p = priv + 9 * i;
+            real *p = priv + 9 * i;
             p[4] = read_guarded(s, n, 4);
             local[2 * get_neighbor(own[i], 5, pitch)] = read_guarded(s, n, 5);
             p[3] = read_guarded(s, get_neighbor(n, 5, s->Hx), 3);
@@ -865,7 +906,7 @@ static void stream_collide_nsol(oracle_sim *s)
         for (unsigned i = 0; i < items; ++i)
         {
             const unsigned n = node[i];
-            double *p = priv + 9 * i;
+            real *p = priv + 9 * i;
             local[2 * get_neighbor(own[i], 7, pitch)] = read_guarded(s, n, 7);
             p[1] = read_guarded(s, get_neighbor(n, 7, s->Hx), 1);
             local[2 * get_neighbor(own[i], 8, pitch) + 1] = read_guarded(s, n, 8);
@@ -915,7 +956,7 @@ void oracle_run(oracle_sim *s, unsigned steps)
             bounce_back(s);
             outlet(s);
             stream_collide_two_lattice(s);
-            double *tmp = s->lat1;
+            real *tmp = s->lat1;
             s->lat1 = s->lat0;
             s->lat0 = tmp;
         }
@@ -993,7 +1034,7 @@ void oracle_reduce(const oracle_sim *s, double out[6])
         double f[9], rho = 0, jx = 0, jy = 0;
         for (int d = 0; d < 9; ++d)
         {
-            f[d] = s->lat0[at(s->layout, n, (unsigned)d, s->N)];
+            f[d] = (double)s->lat0[at(s->layout, n, (unsigned)d, s->N)];
             rho += f[d];
             jx += f[d] * (d % 3 - 1);
             jy += f[d] * (d / 3 - 1);

@@ -76,9 +76,14 @@ void oracle_init(oracle_sim *s);
 /* `steps` reference time steps (execute() loop of the algorithm). */
 void oracle_run(oracle_sim *s, unsigned steps);
 
-/* Current source lattice (9N doubles, layout order) and the 4 result arrays. */
-const double *oracle_populations(const oracle_sim *s);
-void oracle_set_populations(oracle_sim *s, const double *values, size_t count);
+/* Bytes of the build's real_type: 8 (liboracle.so) or 4 (liboracle_f32.so,
+ * the reference's USE_FLOAT build, constants.hpp:17-21). */
+size_t oracle_real_bytes(void);
+
+/* Current source lattice (9N values of real_type, layout order), and the 4
+ * result arrays widened to double. */
+const void *oracle_populations(const oracle_sim *s);
+void oracle_set_populations(oracle_sim *s, const void *values, size_t count);
 void oracle_results(const oracle_sim *s, double *rho, double *jx, double *jy, double *jabs);
 size_t oracle_result_count(const oracle_sim *s);
 

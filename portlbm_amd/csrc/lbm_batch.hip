@@ -290,20 +290,8 @@ int init_domain_lattice(lbm_batch *b, size_t d, typename Storage<S>::T *l0)
     hipLaunchKernelGGL((k_init_rest<LBM_LAYOUT_STREAM, S>), dim3((g.N + 255u) / 256u), dim3(256), 0, b->stream,
                        l0 + dm.lat, g);
     InoutValues v;
-    static const double w[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9,
-                                1.0 / 9,  1.0 / 36, 1.0 / 9, 1.0 / 36};
-    for (int k = 0; k < 9; ++k)
-    {
-        const int cx = k % 3 - 1, cy = k / 3 - 1;
-        const double ui = cx * ds.inlet_velocity_x + cy * ds.inlet_velocity_y;
-        const double uo = cx * ds.outlet_velocity_x + cy * ds.outlet_velocity_y;
-        v.v[k] = w[k] * (ds.inlet_density + 3 * ui + 9.0 / 2 * std::pow(ui, 2) -
-                         3.0 / 2 * (ds.inlet_velocity_x * ds.inlet_velocity_x +
-                                    ds.inlet_velocity_y * ds.inlet_velocity_y));
-        v.v[9 + k] = w[k] * (ds.outlet_density + 3 * uo + 9.0 / 2 * std::pow(uo, 2) -
-                             3.0 / 2 * (ds.outlet_velocity_x * ds.outlet_velocity_x +
-                                        ds.outlet_velocity_y * ds.outlet_velocity_y));
-    }
+    host_equilibrium(S, ds.inlet_velocity_x, ds.inlet_velocity_y, ds.inlet_density, v.v);
+    host_equilibrium(S, ds.outlet_velocity_x, ds.outlet_velocity_y, ds.outlet_density, v.v + 9);
     hipLaunchKernelGGL((k_init_inout<LBM_LAYOUT_STREAM, S>), dim3((dm.V - 2u + 255u) / 256u, 2), dim3(256), 0,
                        b->stream, l0 + dm.lat, dm.H, dm.V, g, dm.H, dm.V, 0, v);
     BATCH_TRY(hipGetLastError());
@@ -403,7 +391,7 @@ int lbm_batch_create(const lbm_batch_domain *domains, uint32_t count, int32_t st
         dm.code = uint32_t(b->code_nodes);
         dm.res = uint32_t(b->res_doubles);
         dm.uy = uint32_t(b->uy_doubles);
-        dm.omega = 1 / ds.relaxation_time;
+        dm.omega = host_omega(b->storage, ds.relaxation_time);
         dm.rho_out = ds.outlet_density;
         b->phase_off[d] = b->phase_nodes;
         b->arena_elems += 9ull * dm.S;

@@ -12,11 +12,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+#include <type_traits>
+
 #include "../../include/lbm_hip.h"
 
 namespace lbm_dev {
 
-// ---- D2Q9 lattice (core/constants.hpp:31-39, core/access.hpp:203-213) ----
+// ---- D2Q9 lattice (core/constants.hpp:31-39, core/access.hpp:142-152) ----
 // direction d: cx = d % 3 - 1, cy = d / 3 - 1, opposite = 8 - d.
 __device__ __forceinline__ constexpr int cx_of(int d) { return d % 3 - 1; }
 __device__ __forceinline__ constexpr int cy_of(int d) { return d / 3 - 1; }
@@ -28,7 +31,7 @@ __device__ __forceinline__ C weight(int d)
     return (d == 4) ? C(4.0 / 9) : ((d & 1) ? C(1.0 / 9) : C(1.0 / 36));
 }
 
-// ---- population layouts (core/access.hpp:117-171) ----------------------
+// ---- population layouts (core/access.hpp:56-60, 81-85, 106-110) ---------
 template <int L>
 __device__ __forceinline__ uint32_t at(uint32_t node, uint32_t d, uint32_t total)
 {
@@ -42,6 +45,13 @@ __device__ __forceinline__ uint32_t at(uint32_t node, uint32_t d, uint32_t total
 
 // ---- storage types ------------------------------------------------------
 // T: what lives in HBM; C: the arithmetic type of the collision.
+// kPromote: the arithmetic of the reference's USE_FLOAT build
+// (core/constants.hpp:17-21).  Its kernels keep their double literals, so in
+// the equilibrium the terms 9.0 / 2 * c.j * c.j and 3.0 / 2 * |j|^2 are
+// double, the sum is rounded to float once (nptl_kernels.hpp:401-402), and
+// the outlet's rebuilt populations are computed in double and rounded once
+// (non_buffered.hpp:117-129).  FP32 storage follows that build bit for bit;
+// the 16-bit forms have no reference counterpart and compute in float.
 template <int S>
 struct Storage;
 
@@ -50,6 +60,7 @@ struct Storage<LBM_STORE_F64>
 {
     using T = double;
     using C = double;
+    static constexpr bool kPromote = false;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
     static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
     static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
@@ -62,6 +73,7 @@ struct Storage<LBM_STORE_F32>
 {
     using T = float;
     using C = float;
+    static constexpr bool kPromote = true;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
     static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
     static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
@@ -101,6 +113,7 @@ struct Storage<LBM_STORE_F16_SHIFTED>
 {
     using T = _Float16;
     using C = float;
+    static constexpr bool kPromote = false;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
     {
         return float(p[i]) + weight<float>(d);
@@ -152,6 +165,7 @@ struct Storage<LBM_STORE_BF16_SHIFTED>
 {
     using T = uint16_t;
     using C = float;
+    static constexpr bool kPromote = false;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
     {
         return bf16_to_f32(p[i]) + weight<float>(d);
@@ -207,7 +221,7 @@ struct StepParams
     double *jabs;
     const double *uy_in;  // outlet uy history, read (indexed by unexpanded y)
     double *uy_out;       // outlet uy history, written this step
-    double omega;         // 1 / tau, computed on the host in double
+    double omega;         // 1 / tau in the storage's real type (host_omega)
     double rho_out;
     uint32_t H, V;        // unexpanded extents incl. ghosts
     uint32_t Hx, Vx;      // expanded extents
@@ -255,15 +269,17 @@ __device__ __forceinline__ void pull(const typename Storage<S>::T *src, uint32_t
 // Zou-He pressure outlet at x = H-2 (non_buffered.hpp:95-129).  The three
 // populations entering from the ghost column (0, 3, 6) are rebuilt from the
 // six known post-stream populations, the outlet density and the previous
-// step's uy of column H-3.
-template <class C>
+// step's uy of column H-3.  A: the type of the rebuild (double under kPromote).
+template <class C, bool kPromote = false>
 __device__ __forceinline__ void zou_he_outlet(C f[9], C rho_out, C uy_prev)
 {
+    using A = std::conditional_t<kPromote, double, C>;
     const C jx = -rho_out + (f[1] + f[4] + f[7] + C(2) * (f[8] + f[5] + f[2]));
     const C f8 = f[8], f5 = f[5], f2 = f[2], f1 = f[1], f7 = f[7];
-    f[0] = f8 - C(0.5) * (f1 - f7) - C(1.0 / 6) * jx - C(0.5) * uy_prev;
-    f[3] = f5 - C(2.0 / 3) * jx;
-    f[6] = f2 + C(0.5) * (f1 - f7) - C(1.0 / 6) * jx + C(0.5) * uy_prev;
+    const A d17 = A(f1 - f7), ax = A(jx), auy = A(uy_prev);
+    f[0] = C(A(f8) - A(0.5) * d17 - A(1.0 / 6) * ax - A(0.5) * auy);
+    f[3] = C(A(f5) - A(2.0 / 3) * ax);
+    f[6] = C(A(f2) + A(0.5) * d17 - A(1.0 / 6) * ax + A(0.5) * auy);
 }
 
 // Moments (rho = sum f, j = sum f c) and BGK relaxation towards the
@@ -305,22 +321,25 @@ __device__ __forceinline__ C c_dot(int d, C jx, C jy)
         return C(cy_of(d)) * jy;
 }
 
-template <class C>
+// A: the type of the equilibrium's quadratic terms and of its sum (double
+// under kPromote, rounded to C once per direction).
+template <class C, bool kPromote = false>
 __device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C &usq)
 {
+    using A = std::conditional_t<kPromote, double, C>;
     moments<C>(f, rho, jx, jy, usq);
-    const C s = C(1.5) * usq;
+    const A s = A(1.5) * A(usq);
     C feq[9];
 #pragma unroll
     for (int d = 0; d < 4; ++d)
     {
         const C cu = c_dot<C>(d, jx, jy);
         const C t = C(3) * cu;
-        const C q = C(4.5) * cu * cu;
-        feq[d] = weight<C>(d) * (rho + t + q - s);
-        feq[8 - d] = weight<C>(8 - d) * (rho - t + q - s);
+        const A q = A(4.5) * A(cu) * A(cu);
+        feq[d] = C(A(weight<C>(d)) * (A(rho + t) + q - s));
+        feq[8 - d] = C(A(weight<C>(8 - d)) * (A(rho - t) + q - s));
     }
-    feq[4] = weight<C>(4) * (rho - s);
+    feq[4] = C(A(weight<C>(4)) * (A(rho) - s));
 #pragma unroll
     for (int d = 0; d < 9; ++d)
         f[d] = -omega * (f[d] - feq[d]) + f[d];
@@ -456,10 +475,44 @@ __device__ __forceinline__ void finish_node(const StepParams &p, uint32_t code, 
 {
     using C = typename Storage<S>::C;
     if (code & LBM_CODE_OUTLET)
-        zou_he_outlet<C>(f, C(p.rho_out), C(p.uy_in[y]));
+        zou_he_outlet<C, Storage<S>::kPromote>(f, C(p.rho_out), C(p.uy_in[y]));
     C rho, jx, jy, usq;
-    collide<C>(f, C(p.omega), rho, jx, jy, usq);
+    collide<C, Storage<S>::kPromote>(f, C(p.omega), rho, jx, jy, usq);
     write_moments<C>(p, code, x, y, rho, jx, jy, usq);
+}
+
+// ---- host side: the real-type parameters of a step -----------------------
+// relaxation_time_inverse = 1 / relaxation_time in real_type
+// (nptl_kernels.hpp:341); the reference reads tau from the settings as
+// real_type (file_interaction.cpp:58-64), so the FP32 storage divides in float.
+inline double host_omega(int storage, double tau)
+{
+    if (storage == LBM_STORE_F32)
+        return double(1.0f / float(tau));
+    return 1.0 / tau;
+}
+
+// maxwell_boltzmann_distribution (core/simulation.hpp:333-350) in the
+// storage's real type: double, or, for FP32 storage, float arguments with
+// the reference's double literals and std::pow (double), rounded to float.
+// The 16-bit forms take the FP64 values and round them when stored.
+inline void host_equilibrium(int storage, double ux, double uy, double rho, double out[9])
+{
+    static const double w[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9,
+                                1.0 / 9,  1.0 / 36, 1.0 / 9, 1.0 / 36};
+    for (int d = 0; d < 9; ++d)
+    {
+        const int cx = d % 3 - 1, cy = d / 3 - 1;
+        if (storage == LBM_STORE_F32)
+        {
+            const float fx = float(ux), fy = float(uy), fr = float(rho), wf = float(w[d]);
+            const float cu = cx * fx + cy * fy;
+            out[d] = double(float(wf * (fr + 3 * cu + 9.0 / 2 * std::pow(cu, 2) - 3.0 / 2 * (fx * fx + fy * fy))));
+        }
+        else
+            out[d] = w[d] * (rho + 3 * (cx * ux + cy * uy) + 9.0 / 2 * std::pow(cx * ux + cy * uy, 2) -
+                             3.0 / 2 * (ux * ux + uy * uy));
+    }
 }
 
 }  // namespace lbm_dev

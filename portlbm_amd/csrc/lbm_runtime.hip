@@ -93,20 +93,6 @@ struct lbm_ctx
     double feq_in[9], feq_out[9];
 };
 
-// Host equilibrium with the reference's own expression
-// (core/simulation.hpp:333-350, std::pow on the host).
-static void host_equilibrium(double ux, double uy, double rho, double out[9])
-{
-    static const double w[9] = {1.0 / 36, 1.0 / 9, 1.0 / 36, 1.0 / 9, 4.0 / 9,
-                                1.0 / 9,  1.0 / 36, 1.0 / 9, 1.0 / 36};
-    for (int d = 0; d < 9; ++d)
-    {
-        const int cx = d % 3 - 1, cy = d / 3 - 1;
-        out[d] = w[d] * (rho + 3 * (cx * ux + cy * uy) + 9.0 / 2 * std::pow(cx * ux + cy * uy, 2) -
-                         3.0 / 2 * (ux * ux + uy * uy));
-    }
-}
-
 // ---------------------------------------------------------------------------
 // step kernels
 // ---------------------------------------------------------------------------
@@ -1045,7 +1031,7 @@ __global__ void k_split_outlet(StepParams p, double *pass, uint32_t PV, uint32_t
 #pragma unroll
     for (int d = 0; d < 9; ++d)
         f[d] = C(pass[size_t(d) * PV + pn]);
-    zou_he_outlet<C>(f, C(p.rho_out), C(p.uy_in[y]));
+    zou_he_outlet<C, Storage<S>::kPromote>(f, C(p.rho_out), C(p.uy_in[y]));
     pass[pn] = double(f[0]);
     pass[3ull * PV + pn] = double(f[3]);
     pass[6ull * PV + pn] = double(f[6]);
@@ -1089,7 +1075,7 @@ __global__ void k_split_collide(StepParams p, const double *pass, uint32_t PV)
         for (int d = 0; d < 9; ++d)
             f[d] = C(pass[size_t(d) * PV + pn]);
         C rho, jx, jy, usq;
-        collide<C>(f, C(p.omega), rho, jx, jy, usq);
+        collide<C, Storage<S>::kPromote>(f, C(p.omega), rho, jx, jy, usq);
     }
     if (fluid || fill)
         store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr);
@@ -1125,7 +1111,7 @@ struct Impl
         const int parity = int(c->iteration & 1u);
         p.uy_in = c->uycol + size_t(parity) * c->V;
         p.uy_out = c->uycol + size_t(1 - parity) * c->V;
-        p.omega = 1 / c->desc.relaxation_time;
+        p.omega = host_omega(c->desc.storage, c->desc.relaxation_time);
         p.rho_out = c->desc.outlet_density;
         p.H = c->H;
         p.V = c->V;
@@ -1575,8 +1561,9 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
         c->Sd = uint32_t(Sd);
         c->lattice_elems = size_t(9) * Sd;
     }
-    host_equilibrium(desc->inlet_velocity_x, desc->inlet_velocity_y, desc->inlet_density, c->feq_in);
-    host_equilibrium(desc->outlet_velocity_x, desc->outlet_velocity_y, desc->outlet_density, c->feq_out);
+    host_equilibrium(desc->storage, desc->inlet_velocity_x, desc->inlet_velocity_y, desc->inlet_density, c->feq_in);
+    host_equilibrium(desc->storage, desc->outlet_velocity_x, desc->outlet_velocity_y, desc->outlet_density,
+                     c->feq_out);
 
     auto cleanup = [&](int rc) {
         lbm_destroy(c);

@@ -8,6 +8,17 @@ from oracle import binding as ob
 
 # Plain FP64 path: populations, density and velocity within 1e-12 relative.
 FP64_RTOL = 1e-12
+# FP32 storage against the oracle's float instance (the reference's USE_FLOAT
+# build): the same operations in the same types and order.
+FP32_RTOL = 1e-6
+
+
+def oracle_real(doc: dict) -> str:
+    """The oracle instance a settings document is checked against."""
+    storage = doc.get("mi355x", {}).get("storage", "f64")
+    if storage not in ("f64", "f32"):
+        raise ValueError(f"no oracle instance for storage {storage}")
+    return storage
 
 
 def fluid_slots(phase: np.ndarray, layout: str, N: int) -> np.ndarray:
@@ -41,7 +52,7 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
 def run_pair(doc: dict, steps: int):
     """Run the same settings on the GPU library and on the oracle."""
     lat = pa.Lattice.from_settings(doc)
-    sim = ob.Simulation(doc)
+    sim = ob.Simulation(doc, real=oracle_real(doc))
     gpu_phase = lat.phase()
     if steps:
         lat.run(steps, doc["algorithmic"]["frameUpdateInterval"])
@@ -49,7 +60,9 @@ def run_pair(doc: dict, steps: int):
     return lat, sim, gpu_phase
 
 
-def compare(doc: dict, steps: int, rtol: float = FP64_RTOL) -> dict:
+def compare(doc: dict, steps: int, rtol: float | None = None) -> dict:
+    if rtol is None:
+        rtol = FP64_RTOL if oracle_real(doc) == "f64" else FP32_RTOL
     lat, sim, gpu_phase = run_pair(doc, steps)
     try:
         ophase = sim.phase()

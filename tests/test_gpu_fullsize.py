@@ -1,0 +1,134 @@
+"""The measured workloads at their own sizes on the GPU, through the C ABI.
+
+* Porous channel 8192 x 2048 (seeded discs, radii U[4, 16), porosity 0.7,
+  tau 0.6, inlet 1.05 / outlet 1.0): every algorithm x layout against the
+  oracle after 20 steps -- FP64 against the FP64 instance (1e-12 relative),
+  FP32 against the float instance (the reference's USE_FLOAT build).
+* The same channel in FP32 and 16-bit storage against the GPU's FP64 after
+  500 steps: relative L2 of density and momentum.
+* Cylinder channel 8192 x 2048 (r 64 at x 1024), tau 0.6 and 0.8, inlet
+  1.02 / outlet 1.0: FP64 against the oracle after 20 steps; 400 steps in
+  every storage, finite populations and bounded L2 / mass difference against
+  FP64.
+* 512 uneven channels (lbm_batch): persistent and graph launches equal
+  per-domain launches bit for bit after 200 steps; sampled domains equal the
+  oracle (FP32: the float instance, bit for bit).
+"""
+import numpy as np
+import pytest
+
+import portlbm_amd as pa
+from oracle import binding as ob
+from portlbm_amd import workloads
+from tests.parity import FP32_RTOL, FP64_RTOL, compare
+
+pytestmark = pytest.mark.gpu
+
+ALGS = ["nptl", "lptl", "npol", "nsol"]
+LAYOUTS = ["collision", "stream", "bundle"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("alg", ALGS)
+def test_porous_8192x2048_fp64_matches_oracle(alg, layout):
+    doc = workloads.porous_channel(alg, layout, "f64", steps=20)
+    r = compare(doc, 20)
+    print("porous 8192x2048 f64", alg, layout, r, flush=True)
+    assert r["phase_equal"] and r["codes_equal"]
+    assert r["pop_err"] <= FP64_RTOL and r["field_err"] <= FP64_RTOL
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("alg", ALGS)
+def test_porous_8192x2048_fp32_matches_float_oracle(alg, layout):
+    doc = workloads.porous_channel(alg, layout, "f32", steps=20)
+    r = compare(doc, 20)
+    print("porous 8192x2048 f32", alg, layout, r, flush=True)
+    assert r["phase_equal"] and r["codes_equal"]
+    assert r["pop_err"] <= FP32_RTOL and r["field_err"] <= FP32_RTOL
+
+
+def _fields(doc, steps):
+    """Fields after `steps`; `finite` from the FP64 sums of the fluid
+    populations themselves (the reported fields clamp NaN to 0)."""
+    with pa.Lattice.from_settings(doc) as lat:
+        lat.run(steps)
+        rho, jx, jy, _ = lat.macros()
+        red = lat.reduce()
+    finite = bool(np.isfinite([red["mass"], red["jx"], red["jy"]]).all())
+    return rho, jx, jy, red, finite
+
+
+def _l2(a, b, mask):
+    return float(np.linalg.norm(a[mask] - b[mask]) / np.linalg.norm(b[mask]))
+
+
+# bounds: about 3x the values measured on the first full run
+POROUS_BOUNDS = {"f32": (1e-6, 1e-4), "f16-shifted": (2e-5, 3e-3), "bf16-shifted": (2e-4, 3e-2)}
+
+
+def test_porous_8192x2048_reduced_storage_against_fp64():
+    steps = 500
+    rho0, jx0, jy0, red0, ok0 = _fields(workloads.porous_channel("nptl", "stream", "f64", steps=steps), steps)
+    assert ok0
+    fluid = rho0 > -1
+    j0 = np.hypot(jx0, jy0)
+    for storage, (rho_bound, u_bound) in POROUS_BOUNDS.items():
+        rho, jx, jy, red, ok = _fields(workloads.porous_channel("nptl", "stream", storage, steps=steps), steps)
+        e_rho = _l2(rho, rho0, fluid)
+        e_u = float(np.linalg.norm(np.hypot(jx - jx0, jy - jy0)[fluid]) / np.linalg.norm(j0[fluid]))
+        dm = abs(red["mass"] - red0["mass"]) / red0["mass"]
+        print(f"porous 8192x2048 {storage} vs f64 after {steps}: rho L2 {e_rho:.3e} u L2 {e_u:.3e} "
+              f"mass {dm:.3e}", flush=True)
+        assert ok and e_rho <= rho_bound and e_u <= u_bound and dm <= rho_bound
+
+
+@pytest.mark.parametrize("tau", [0.6, 0.8])
+def test_cylinder_8192x2048_fp64_matches_oracle(tau):
+    r = compare(workloads.cylinder_channel(tau, "f64", steps=20), 20)
+    print("cylinder 8192x2048 f64 tau", tau, r, flush=True)
+    assert r["ok"], r
+
+
+CYL_BOUNDS = {"f32": (1e-6, 1e-4), "f16-shifted": (2e-5, 5e-3), "bf16-shifted": (2e-4, 5e-2)}
+
+
+@pytest.mark.parametrize("tau", [0.6, 0.8])
+def test_cylinder_8192x2048_storages_against_fp64(tau):
+    steps = 400
+    rho0, jx0, jy0, red0, ok0 = _fields(workloads.cylinder_channel(tau, "f64", steps=steps), steps)
+    assert ok0
+    fluid = rho0 > -1
+    j0 = np.hypot(jx0, jy0)
+    for storage, (rho_bound, u_bound) in CYL_BOUNDS.items():
+        rho, jx, jy, red, ok = _fields(workloads.cylinder_channel(tau, storage, steps=steps), steps)
+        e_rho = _l2(rho, rho0, fluid)
+        e_u = float(np.linalg.norm(np.hypot(jx - jx0, jy - jy0)[fluid]) / np.linalg.norm(j0[fluid]))
+        dm = abs(red["mass"] - red0["mass"]) / red0["mass"]
+        print(f"cylinder tau {tau} {storage} vs f64 after {steps}: rho L2 {e_rho:.3e} u L2 {e_u:.3e} "
+              f"mass {dm:.3e}", flush=True)
+        assert ok and e_rho <= rho_bound and e_u <= u_bound and dm <= rho_bound
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_batch_512_channels_modes_and_oracle(storage):
+    steps = 200
+    docs = workloads.batch_channels(512, storage=storage, steps=steps)
+    results = {}
+    for mode in ("per-domain", "persistent", "graph"):
+        with pa.Batch(docs, storage=storage) as b:
+            b.run(steps, mode)
+            results[mode] = [b.macros(i) for i in range(len(docs))]
+        print(f"batch 512 {storage} {mode} done", flush=True)
+    for mode in ("persistent", "graph"):
+        for i in range(len(docs)):
+            for a, r in zip(results[mode][i], results["per-domain"][i]):
+                assert np.array_equal(a, r), (mode, i)
+    rtol = FP32_RTOL if storage == "f32" else FP64_RTOL
+    for i in (0, 1, 255, 511):
+        s = ob.Simulation(docs[i], real=storage)
+        s.run(steps)
+        for a, r in zip(results["graph"][i], s.results()):
+            den = np.maximum(np.abs(r), 1e-300)
+            assert float(np.max(np.abs(a - r) / den)) <= rtol, (storage, i)
+        s.close()

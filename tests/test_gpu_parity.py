@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import portlbm_amd as pa
-from tests.parity import FP64_RTOL, compare
+from tests.parity import FP32_RTOL, FP64_RTOL, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -17,18 +17,21 @@ ALGS = ["nptl", "lptl", "npol", "nsol"]
 LAYOUTS = ["collision", "stream", "bundle"]
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32"])
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("alg", ALGS)
-def test_hagen_poiseuille_256x64_1000_steps(alg, layout):
-    """256 x 64 Hagen-Poiseuille, tau 0.6, inlet 1.2 / outlet 1.0, 1000 steps."""
+def test_hagen_poiseuille_256x64_1000_steps(alg, layout, storage):
+    """256 x 64 Hagen-Poiseuille, tau 0.6, inlet 1.2 / outlet 1.0, 1000 steps;
+    FP64 against the oracle, FP32 against its float instance (USE_FLOAT)."""
     doc = pa.settings(alg, layout, inner_h=256, inner_v=64, tau=0.6, rho_in=1.2, rho_out=1.0,
-                      time_steps=1000, work_group_size=256)
+                      time_steps=1000, work_group_size=256, storage=storage)
     r = compare(doc, 1000)
-    print(alg, layout, r, flush=True)
+    print(alg, layout, storage, r, flush=True)
+    rtol = FP64_RTOL if storage == "f64" else FP32_RTOL
     assert r["phase_equal"] and r["codes_equal"]
     assert r["fluid_nodes"] == 256 * 62
-    assert r["pop_err"] <= FP64_RTOL
-    assert r["field_err"] <= FP64_RTOL
+    assert r["pop_err"] <= rtol
+    assert r["field_err"] <= rtol
 
 
 # Ragged shapes: padding in nptl, partial tiles in npol/nsol, odd and
@@ -59,6 +62,19 @@ def test_ragged_shapes(alg, w, h, wg, layout):
                       scenario="circle", tau=0.7, rho_in=1.05)
     r = compare(doc, 300)
     print(alg, w, h, wg, layout, r, flush=True)
+    assert r["ok"], r
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("alg,w,h,wg", RAGGED)
+def test_ragged_shapes_fp32(alg, w, h, wg, layout):
+    """The same shapes in FP32 storage against the oracle's float instance,
+    at tau 0.52 (the smallest relaxation time of the measured workloads: the
+    float 1 / tau differs from 1 / tau rounded from double there)."""
+    doc = pa.settings(alg, layout, inner_h=w, inner_v=h, work_group_size=wg, time_steps=300,
+                      scenario="circle", tau=0.52, rho_in=1.05, storage="f32")
+    r = compare(doc, 300)
+    print(alg, w, h, wg, layout, "f32", r, flush=True)
     assert r["ok"], r
 
 

@@ -142,15 +142,170 @@ def test_rest_state_is_a_fixed_point(alg):
     s.close()
 
 
-def test_equilibrium_moments_closed_form():
-    """feq of the host expression reproduces (rho, j) exactly in its moments."""
-    rng = np.random.default_rng(3)
-    for _ in range(20):
-        rho, ux, uy = 1 + 0.1 * rng.standard_normal(), 0.05 * rng.standard_normal(), 0.05 * rng.standard_normal()
-        cu = CX * ux + CY * uy
-        feq = W * (rho + 3 * cu + 4.5 * cu ** 2 - 1.5 * (ux * ux + uy * uy))
-        assert abs(feq.sum() - rho) < 1e-14
-        assert abs((feq * CX).sum() - ux) < 1e-15 and abs((feq * CY).sum() - uy) < 1e-15
+# ---- the BGK collision, pinned by what the mathematics fixes ----
+#
+# A spatially uniform state is left unchanged by streaming, so after one step
+# every node away from walls, inlet and outlet holds collide(g) of the uniform
+# populations g.  The nine moments M = sum_d f_d phi_d(c_d) of D2Q9 (1, cx,
+# cy, cx^2, cy^2, cx cy, cx^2 cy, cx cy^2, cx^2 cy^2) determine f; BGK
+# relaxes each towards its equilibrium value with the same factor,
+# M' = (1 - 1/tau) M + (1/tau) M_eq, and the lattice's weights fix M_eq of the
+# incompressible equilibrium in closed form (sums of w_d c^k over the nine
+# velocities): rho, jx, jy, rho/3 + jx^2, rho/3 + jy^2, jx jy, jy/3, jx/3,
+# rho/9 + |j|^2/3.  None of this restates feq_d: the check holds for any
+# collision that is BGK towards that equilibrium and fails otherwise.
+
+
+def _moment_basis():
+    return np.stack([np.ones(9), CX, CY, CX ** 2, CY ** 2, CX * CY, CX ** 2 * CY, CX * CY ** 2,
+                     CX ** 2 * CY ** 2]).astype(np.float64)
+
+
+def _moments_eq(rho, jx, jy):
+    return np.array([rho, jx, jy, rho / 3 + jx * jx, rho / 3 + jy * jy, jx * jy, jy / 3, jx / 3,
+                     rho / 9 + (jx * jx + jy * jy) / 3])
+
+
+def _collision_moment_error(tau, real="f64", lib_path=None, seed=0, alg="nptl", layout="stream"):
+    """max |M'(node) - ((1 - 1/tau) M + M_eq / tau)| over the interior nodes
+    after one step from a uniform random state; also the error of the
+    reported (pre-collision) macroscopic values."""
+    inner_h, inner_v = 24, 16
+    doc = pa.settings(alg, layout, inner_h=inner_h, inner_v=inner_v, work_group_size=64, time_steps=1, tau=tau,
+                      rho_in=1.0, rho_out=1.0)
+    s = ob.Simulation(doc, real=real, lib_path=lib_path)
+    rng = np.random.default_rng(seed)
+    g = W * (1 + 0.6 * (rng.random(9) - 0.5))  # |j| ~ 0.05-0.1
+    g = g.astype(s.dtype).astype(np.float64)
+    N = s.N
+    vals = np.empty(9 * N)
+    nodes = np.arange(N)
+    for d in range(9):
+        vals[ob_layout_index(layout, nodes, d, N)] = g[d]
+    s.set_populations(vals)
+    s.run(1)
+    f = s.populations().astype(np.float64)
+    rho_r, jx_r, jy_r, jabs_r = s.results()
+    Hx = s.extents["Hx"]
+    H, V = inner_h + 2, inner_v + 2
+    s.close()
+    B = _moment_basis()
+    m0 = B @ g
+    rho, jx, jy = m0[0], m0[1], m0[2]
+    expect = (1 - 1 / tau) * m0 + _moments_eq(rho, jx, jy) / tau
+    err = 0.0
+    # walls at rows 1 and V-2 (bounce-back next to them), outlet at x = H-2
+    for y in range(3, V - 3):
+        for x in range(1, H - 3):
+            n = x + y * Hx  # nptl/lptl: expanded == unexpanded coordinates
+            fn = np.array([f[ob_layout_index(layout, n, d, N)] for d in range(9)])
+            err = max(err, float(np.abs(B @ fn - expect).max()))
+            r = (x - 1) + (y - 1) * (H - 2)
+            err = max(err, abs(rho_r[r] - rho), abs(jx_r[r] - jx), abs(jy_r[r] - jy),
+                      abs(jabs_r[r] - np.hypot(jx, jy)))
+    return err
+
+
+def ob_layout_index(layout, n, d, N):
+    """The reference's population index (access.hpp:56-60, 81-85, 106-110)."""
+    if layout == "collision":
+        return 9 * n + d
+    if layout == "stream":
+        return N * d + n
+    return 3 * (d // 3) * N + (d % 3) + 3 * n
+
+
+@pytest.mark.parametrize("tau", [0.6, 0.8, 1.3])
+@pytest.mark.parametrize("alg,layout", [("nptl", "stream"), ("lptl", "collision"), ("nptl", "bundle")])
+def test_collision_relaxes_all_nine_moments_fp64(tau, alg, layout):
+    for seed in range(3):
+        assert _collision_moment_error(tau, "f64", seed=seed, alg=alg, layout=layout) < 1e-15
+
+
+@pytest.mark.parametrize("tau", [0.52, 0.6, 0.8, 1.3])
+def test_collision_relaxes_all_nine_moments_fp32_instance(tau):
+    """The USE_FLOAT instance: the same law within float rounding."""
+    for seed in range(3):
+        assert _collision_moment_error(tau, "f32", seed=seed) < 5e-7
+
+
+def _poiseuille_peak_ratio(tau, real="f64", lib_path=None):
+    """Hagen-Poiseuille channel to steady state: centre-line momentum over
+    G W^2 / (8 nu), nu = (tau - 1/2) / 3, G = -dp/dx with p = rho / 3 read off
+    the middle third of the channel; W = 12 fluid rows between half-way walls."""
+    inner_h, inner_v, steps = 60, 14, 40000
+    doc = pa.settings("nptl", "stream", inner_h=inner_h, inner_v=inner_v, work_group_size=64, time_steps=steps,
+                      tau=tau, rho_in=1.01, rho_out=1.0)
+    s = ob.Simulation(doc, real=real, lib_path=lib_path)
+    s.run(steps)
+    rho, jx, _, _ = s.results()
+    s.close()
+    rho, jx = rho.reshape(inner_v, inner_h), jx.reshape(inner_v, inner_h)
+    xs = np.arange(inner_h // 3, 2 * inner_h // 3)
+    G = -np.polyfit(xs, rho[1:-1, xs].mean(axis=0), 1)[0] / 3
+    nu = (tau - 0.5) / 3
+    return float(jx[1:-1, inner_h // 2].max() / (G * (inner_v - 2) ** 2 / (8 * nu)))
+
+
+@pytest.mark.parametrize("tau", [0.6, 0.8])
+def test_poiseuille_peak_follows_the_bgk_viscosity(tau):
+    """Measured 0.986 (tau 0.6) and 0.989 (tau 0.8): the rest is the known
+    slip of half-way bounce-back at W = 12."""
+    assert abs(_poiseuille_peak_ratio(tau) - 1) < 0.03
+
+
+@pytest.fixture(scope="module")
+def planted_oracles(tmp_path_factory):
+    """oracle.c built with each deliberate collision mistake (OR_PLANT)."""
+    import subprocess
+
+    src = Path(ob.__file__).resolve().parent / "oracle.c"
+    out = {}
+    d = tmp_path_factory.mktemp("planted")
+    for k in (1, 2, 3, 4):
+        so = d / f"liboracle_plant{k}.so"
+        subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", f"-DOR_PLANT={k}",
+                        "-Wl,-Bsymbolic", "-o", str(so), str(src), "-lm"], check=True)
+        out[k] = so
+    return out
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3, 4])
+def test_planted_collision_mistakes_are_caught(planted_oracles, plant):
+    """omega = tau, 3.0 for 9/2, 1.0 for 3/2, a dropped 3 c.j: the moment
+    check fails each by more than 1e-4 (measured 3e-3 to 8e-2; correct: 6e-17)."""
+    assert _collision_moment_error(0.6, lib_path=planted_oracles[plant]) > 1e-4
+
+
+def test_planted_relaxation_mistake_fails_the_poiseuille_check(planted_oracles):
+    assert abs(_poiseuille_peak_ratio(0.8, lib_path=planted_oracles[1]) - 1) > 0.5
+
+
+def test_reduction_sums_and_momentum_exchange_independently():
+    """oracle_reduce: mass and momentum equal the sums of the reported
+    (pre-collision) fields, which collision conserves; the momentum-exchange
+    force equals 2 f_d c_d summed over every fluid-to-solid link found from
+    the phase table's geometry (not from the node codes)."""
+    doc = pa.settings("nptl", "stream", inner_h=48, inner_v=20, work_group_size=64, scenario="circle",
+                      time_steps=300, tau=0.7, rho_in=1.05, rho_out=1.0)
+    s = ob.Simulation(doc)
+    s.run(300)
+    red = s.reduce()
+    rho, jx, jy, _ = s.results()
+    fluid = rho > -1
+    assert red["fluid"] == int(fluid.sum())
+    assert abs(red["mass"] - rho[fluid].sum()) < 1e-11
+    assert abs(red["jx"] - jx[fluid].sum()) < 1e-13 and abs(red["jy"] - jy[fluid].sum()) < 1e-13
+    phase, f, N, Hx = s.phase(), s.populations(), s.N, s.extents["Hx"]
+    fx = fy = 0.0
+    for n in np.flatnonzero(phase == 0):
+        for d in range(9):
+            if d != 4 and phase[n + CX[d] + CY[d] * Hx] == 1:
+                fx += 2 * f[N * d + n] * CX[d]
+                fy += 2 * f[N * d + n] * CY[d]
+    s.close()
+    assert abs(red["fx"] - fx) < 1e-13 and abs(red["fy"] - fy) < 1e-13
+    assert red["fx"] > 0  # the flow pushes the disc and the walls downstream
 
 
 def test_outlet_closed_form_after_one_step():
