@@ -52,6 +52,11 @@ __device__ __forceinline__ uint32_t at(uint32_t node, uint32_t d, uint32_t total
 // the outlet's rebuilt populations are computed in double and rounded once
 // (non_buffered.hpp:117-129).  FP32 storage follows that build bit for bit;
 // the 16-bit forms have no reference counterpart and compute in float.
+// kFull: the moment and equilibrium sums keep every term of the reference,
+// also those whose velocity component is zero (x + 0 * f).  For finite
+// populations leaving them out changes no result; once a population is
+// infinite, 0 * inf is NaN in the reference, and the full sums propagate a
+// blow-up exactly as it does (FP64, FP32).  The 16-bit forms leave them out.
 template <int S>
 struct Storage;
 
@@ -61,6 +66,7 @@ struct Storage<LBM_STORE_F64>
     using T = double;
     using C = double;
     static constexpr bool kPromote = false;
+    static constexpr bool kFull = true;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
     static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
     static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
@@ -74,6 +80,7 @@ struct Storage<LBM_STORE_F32>
     using T = float;
     using C = float;
     static constexpr bool kPromote = true;
+    static constexpr bool kFull = true;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
     static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
     static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
@@ -114,6 +121,7 @@ struct Storage<LBM_STORE_F16_SHIFTED>
     using T = _Float16;
     using C = float;
     static constexpr bool kPromote = false;
+    static constexpr bool kFull = false;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
     {
         return float(p[i]) + weight<float>(d);
@@ -166,6 +174,7 @@ struct Storage<LBM_STORE_BF16_SHIFTED>
     using T = uint16_t;
     using C = float;
     static constexpr bool kPromote = false;
+    static constexpr bool kFull = false;
     static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
     {
         return bf16_to_f32(p[i]) + weight<float>(d);
@@ -285,13 +294,13 @@ __device__ __forceinline__ void zou_he_outlet(C f[9], C rho_out, C uy_prev)
 // Moments (rho = sum f, j = sum f c) and BGK relaxation towards the
 // incompressible equilibrium feq = w (rho + 3 c.j + 4.5 (c.j)^2 - 1.5 |j|^2);
 // f' = -(1/tau)(f - feq) + f.  Velocity is the momentum j (never divided by
-// rho).  Term order as in nptl_kernels.hpp:382-405, with two rewrites that
-// leave every finite result bit for bit unchanged (at most the sign of an
-// exact zero in j differs): terms with a zero velocity component are left
-// out (x + 0 * f == x; without fast-math the compiler may not drop them), and
-// a direction and its opposite share 3 c.j and 4.5 (c.j)^2, since c.j of the
-// opposite is the exact negation and (4.5 (-a)) (-a) == (4.5 a) a.
-template <class C>
+// rho).  Term order as in nptl_kernels.hpp:382-405.  A direction and its
+// opposite share 3 c.j and 4.5 (c.j)^2: c.j of the opposite is the exact
+// negation of c.j for every value, infinities and NaN included, and
+// (4.5 (-a)) (-a) == (4.5 a) a.  Without kFull the terms with a zero
+// velocity component are left out as well (x + 0 * f == x for finite f;
+// without fast-math the compiler may not drop them itself).
+template <class C, bool kFull = true>
 __device__ __forceinline__ void moments(const C f[9], C &rho, C &jx, C &jy, C &usq)
 {
     rho = C(0);
@@ -301,19 +310,19 @@ __device__ __forceinline__ void moments(const C f[9], C &rho, C &jx, C &jy, C &u
     for (int d = 0; d < 9; ++d)
     {
         rho += f[d];
-        if (cx_of(d) != 0)
+        if (kFull || cx_of(d) != 0)
             jx += f[d] * C(cx_of(d));
-        if (cy_of(d) != 0)
+        if (kFull || cy_of(d) != 0)
             jy += f[d] * C(cy_of(d));
     }
     usq = jx * jx + jy * jy;
 }
 
-// c_d . j with the reference's term order, zero components left out
-template <class C>
+// c_d . j with the reference's term order (zero components left out unless kFull)
+template <class C, bool kFull = true>
 __device__ __forceinline__ C c_dot(int d, C jx, C jy)
 {
-    if (cx_of(d) != 0 && cy_of(d) != 0)
+    if (kFull || (cx_of(d) != 0 && cy_of(d) != 0))
         return C(cx_of(d)) * jx + C(cy_of(d)) * jy;
     else if (cx_of(d) != 0)
         return C(cx_of(d)) * jx;
@@ -323,23 +332,31 @@ __device__ __forceinline__ C c_dot(int d, C jx, C jy)
 
 // A: the type of the equilibrium's quadratic terms and of its sum (double
 // under kPromote, rounded to C once per direction).
-template <class C, bool kPromote = false>
+template <class C, bool kPromote = false, bool kFull = true>
 __device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C &usq)
 {
     using A = std::conditional_t<kPromote, double, C>;
-    moments<C>(f, rho, jx, jy, usq);
+    moments<C, kFull>(f, rho, jx, jy, usq);
     const A s = A(1.5) * A(usq);
     C feq[9];
 #pragma unroll
     for (int d = 0; d < 4; ++d)
     {
-        const C cu = c_dot<C>(d, jx, jy);
+        const C cu = c_dot<C, kFull>(d, jx, jy);
         const C t = C(3) * cu;
         const A q = A(4.5) * A(cu) * A(cu);
         feq[d] = C(A(weight<C>(d)) * (A(rho + t) + q - s));
         feq[8 - d] = C(A(weight<C>(8 - d)) * (A(rho - t) + q - s));
     }
-    feq[4] = C(A(weight<C>(4)) * (A(rho) - s));
+    if constexpr (kFull)
+    {
+        const C cu = c_dot<C, true>(4, jx, jy);  // 0 jx + 0 jy
+        const C t = C(3) * cu;
+        const A q = A(4.5) * A(cu) * A(cu);
+        feq[4] = C(A(weight<C>(4)) * (A(rho + t) + q - s));
+    }
+    else
+        feq[4] = C(A(weight<C>(4)) * (A(rho) - s));
 #pragma unroll
     for (int d = 0; d < 9; ++d)
         f[d] = -omega * (f[d] - feq[d]) + f[d];
@@ -477,7 +494,7 @@ __device__ __forceinline__ void finish_node(const StepParams &p, uint32_t code, 
     if (code & LBM_CODE_OUTLET)
         zou_he_outlet<C, Storage<S>::kPromote>(f, C(p.rho_out), C(p.uy_in[y]));
     C rho, jx, jy, usq;
-    collide<C, Storage<S>::kPromote>(f, C(p.omega), rho, jx, jy, usq);
+    collide<C, Storage<S>::kPromote, Storage<S>::kFull>(f, C(p.omega), rho, jx, jy, usq);
     write_moments<C>(p, code, x, y, rho, jx, jy, usq);
 }
 

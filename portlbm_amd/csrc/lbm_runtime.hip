@@ -1054,7 +1054,7 @@ __global__ void k_split_moments(StepParams p, double *pass, uint32_t PV, uint32_
     for (int d = 0; d < 9; ++d)
         f[d] = C(pass[size_t(d) * PV + pn]);
     C rho, jx, jy, usq;
-    moments<C>(f, rho, jx, jy, usq);
+    moments<C, Storage<S>::kFull>(f, rho, jx, jy, usq);
     write_moments<C>(p, code, x, y, rho, jx, jy, usq);
 }
 
@@ -1075,7 +1075,7 @@ __global__ void k_split_collide(StepParams p, const double *pass, uint32_t PV)
         for (int d = 0; d < 9; ++d)
             f[d] = C(pass[size_t(d) * PV + pn]);
         C rho, jx, jy, usq;
-        collide<C, Storage<S>::kPromote>(f, C(p.omega), rho, jx, jy, usq);
+        collide<C, Storage<S>::kPromote, Storage<S>::kFull>(f, C(p.omega), rho, jx, jy, usq);
     }
     if (fluid || fill)
         store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr);

@@ -36,13 +36,14 @@ def fluid_slots(phase: np.ndarray, layout: str, N: int) -> np.ndarray:
 
 
 def rel_err(a: np.ndarray, b: np.ndarray) -> float:
-    """max |a - b| / max(|b|, tiny), elementwise, NaNs must coincide."""
+    """max |a - b| / max(|b|, tiny), elementwise; NaNs must coincide, equal
+    values (infinities included) count as no error."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     nan_a, nan_b = np.isnan(a), np.isnan(b)
     if not np.array_equal(nan_a, nan_b):
         return float("inf")
-    m = ~nan_a
+    m = ~nan_a & (a != b)
     if not m.any():
         return 0.0
     den = np.maximum(np.abs(b[m]), 1e-300)
