@@ -27,10 +27,17 @@ def test_warm_start_needs_fewer_steps_than_rest():
     rest = sg.rest_steps(docs, max_steps=40000)
     warm, info = sg.warm_steps(docs, model, max_steps=40000)
     s = sg._summary(rest, warm)
-    print("rest vs warm:", s, info)
+    # every domain counts: one that has not converged by max_steps counts as
+    # max_steps (it took at least that many), for both starts
+    capped = lambda a: np.where(a < 0, 40000, a)
+    ratio_all = float(np.mean(capped(rest)) / np.mean(capped(warm)))
+    print("rest vs warm:", s, info, "ratio of means over all domains:", ratio_all)
+    assert s["warm_unconverged"] <= s["rest_unconverged"], s
     assert s["both_converged"] >= 40
-    # measured on 1000 held-out domains: 1.32x fewer steps on average (median ratio 1.29)
-    assert s["ratio_of_means"] >= 1.1, s
+    # measured on 1000 held-out domains: 1.32x fewer steps on average (median
+    # ratio 1.29); 1.36x on these 48
+    assert ratio_all >= 1.25, (ratio_all, s)
+    assert s["ratio_of_means"] >= 1.25, s
     assert s["warm_median"] < s["rest_median"]
 
 
