@@ -85,8 +85,9 @@ def _against_fp64(make, steps, bounds, label):
         assert ok and e_rho <= rho_bound and e_u <= u_bound and dm <= rho_bound, (storage, out[storage])
 
 
-# (rho L2, u L2) bounds
-POROUS_BOUNDS = {"f32": (1e-5, 1e-3), "f16-shifted": (1e-4, 1e-2), "bf16-shifted": (1e-3, 1e-1)}
+# (rho L2, u L2) bounds, about 3x the values measured on an MI355X after
+# 500 steps: FP32 6.8e-7 / 7.7e-5, FP16 1.3e-6 / 5.4e-4, BF16 8.4e-6 / 4.1e-3
+POROUS_BOUNDS = {"f32": (2e-6, 2.5e-4), "f16-shifted": (5e-6, 1.6e-3), "bf16-shifted": (3e-5, 1.2e-2)}
 
 
 def test_porous_8192x2048_reduced_storage_against_fp64():
@@ -101,7 +102,10 @@ def test_cylinder_8192x2048_fp64_matches_oracle(tau):
     assert r["ok"], r
 
 
-CYL_BOUNDS = {"f32": (1e-5, 1e-3), "f16-shifted": (1e-4, 1e-2), "bf16-shifted": (1e-3, 1e-1)}
+# measured after 400 steps (tau 0.6 / 0.8): FP32 3.4e-7 / 1.7e-4 and 2.3e-7 /
+# 1.4e-4, FP16 5.2e-7 / 7.7e-4 and 3.6e-6 / 3.1e-3, BF16 9.3e-6 / 9.1e-3 and
+# 2.2e-5 / 2.0e-2; bounds about 3x the larger of each
+CYL_BOUNDS = {"f32": (1e-6, 5e-4), "f16-shifted": (1.1e-5, 1e-2), "bf16-shifted": (7e-5, 6e-2)}
 
 
 @pytest.mark.parametrize("tau", [0.6, 0.8])
