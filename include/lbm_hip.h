@@ -311,6 +311,15 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps_per_trial, double *best_ms_p
  * The first call after init compares against zero and returns 1; a domain
  * whose populations hold NaN or Inf yields NaN. */
 int lbm_batch_residuals(lbm_batch *b, double *out, uint32_t count);
+/* lbm_batch_residuals without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device or host memory in
+ * their loop (serving loops, harnesses that forbid it).  The per-tile
+ * partial sums and per-domain residuals live in a static device array of
+ * 8.5 MiB that each GPU holds until the process ends; batches of more than
+ * 524 288 tiles or 65 536 domains are rejected (LBM_ERR_INVALID_ARGUMENT).
+ * Same values, bit for bit, as lbm_batch_residuals.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU race. */
+int lbm_batch_residuals_noalloc(lbm_batch *b, double *out, uint32_t count);
 /* Domain `domain` back to the at-rest start of lbm_batch_init (populations,
  * inlet/outlet ghosts, outlet history); the other domains are untouched. */
 int lbm_batch_reset_domain(lbm_batch *b, uint32_t domain);

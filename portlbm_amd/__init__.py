@@ -594,6 +594,7 @@ def _batch_sigs() -> ctypes.CDLL:
             "lbm_batch_get_tiles": ([_P, _P], ctypes.c_int),
             "lbm_batch_autotune": ([_P, _U32, _DP], ctypes.c_int),
             "lbm_batch_residuals": ([_P, _P, _U32], ctypes.c_int),
+            "lbm_batch_residuals_noalloc": ([_P, _P, _U32], ctypes.c_int),
             "lbm_batch_reset_domain": ([_P, _U32], ctypes.c_int),
             "lbm_batch_set_equilibrium": ([_P, _U32, _P, _P, _P, _SZ], ctypes.c_int),
         }
@@ -682,6 +683,14 @@ class Batch:
     def residuals(self) -> np.ndarray:
         out = np.empty(len(self.shapes))
         _check_hip(self.lib.lbm_batch_residuals(self._b, _ptr(out), out.size), "lbm_batch_residuals")
+        return out
+
+    def residuals_noalloc(self, out: np.ndarray | None = None) -> np.ndarray:
+        """residuals() through lbm_batch_residuals_noalloc (the library
+        allocates nothing; `out` may be a caller-owned float64 array)."""
+        if out is None:
+            out = np.empty(len(self.shapes))
+        _check_hip(self.lib.lbm_batch_residuals_noalloc(self._b, _ptr(out), out.size), "lbm_batch_residuals_noalloc")
         return out
 
     def reset_domain(self, domain: int) -> None:

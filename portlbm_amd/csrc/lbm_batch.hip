@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -166,6 +167,73 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_residual(const BatchDom 
 
 int width_class(uint32_t inner_w) { return inner_w <= 128 ? 0 : inner_w <= 256 ? 1 : inner_w <= 512 ? 2 : 3; }
 
+// ---- scratch of lbm_batch_residuals_noalloc ---------------------------------
+// A file-scope device array (each GPU holds its own copy for the life of the
+// process): the per-tile partial sums (2 doubles per tile) and the per-domain
+// residuals (1 double per domain).  Every call on a GPU shares that GPU's
+// array; overlapping calls on one GPU race.
+constexpr uint32_t kScratchTiles = 1u << 19;    // 8 MiB of partials
+constexpr uint32_t kScratchDomains = 1u << 16;  // 512 KiB of residuals
+constexpr size_t kScratchDoubles = 2ull * kScratchTiles + kScratchDomains;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Device address of `scratch` on each GPU, looked up once per device by
+// lbm_batch_create with that device current (the array's name is a host-side
+// stand-in, not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_error[kMaxDevices] = {};
+bool g_scratch_resolved[kMaxDevices] = {};
+
+hipError_t resolve_scratch(int dev, double **out)
+{
+    *out = nullptr;
+    if (dev < 0 || dev >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_resolved[dev])
+    {
+        void *ptr = nullptr;
+        g_scratch_error[dev] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = g_scratch_error[dev] == hipSuccess ? static_cast<double *>(ptr) : nullptr;
+        g_scratch_resolved[dev] = true;
+    }
+    *out = g_scratch_ptr[dev];
+    return g_scratch_error[dev];
+}
+
+// Per-domain residuals from the per-tile partials, in the order and with the
+// arithmetic of lbm_batch_residuals' host loop: a domain's tiles are
+// contiguous in the tile list (build_tiles), summed in list order; NaN for a
+// domain whose sums are not finite.
+__global__ void k_batch_residual_domains(const BatchTile *tiles, uint32_t ntiles, uint32_t ndoms,
+                                         const double *partials, double *out)
+{
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ndoms)
+        return;
+    uint32_t lo = 0, hi = ntiles;  // first tile of domain d
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (tiles[mid].dom < d)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    double num = 0.0, den = 0.0;
+    for (uint32_t t = lo; t < ntiles && tiles[t].dom == d; ++t)
+    {
+        num += partials[2ull * t];
+        den += partials[2ull * t + 1];
+    }
+    if (!isfinite(num) || !isfinite(den))
+        out[d] = __builtin_nan("");
+    else
+        out[d] = den > 0.0 ? sqrt(num / den) : (num > 0.0 ? 1.0 : 0.0);
+}
+
 }  // namespace
 
 struct lbm_batch
@@ -197,8 +265,10 @@ struct lbm_batch
     hipGraphExec_t graph = nullptr;
     uint32_t graph_steps = 0;
     int graph_parity = -1;
-    double *snap = nullptr;      // 2 doubles per node place (residual snapshot, lazy)
+    double *snap = nullptr;      // 2 doubles per node place (residual snapshot, allocated by lbm_batch_init)
     bool snap_valid = false;
+    double *scratch = nullptr;   // device address of this GPU's `scratch` (null if the lookup failed)
+    hipError_t scratch_error = hipSuccess;
 };
 
 namespace
@@ -432,6 +502,7 @@ int lbm_batch_create(const lbm_batch_domain *domains, uint32_t count, int32_t st
     CREATE_TRY(hipGetDevice(&dev));
     CREATE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     b->persistent_grid = uint32_t(std::max(1, cus)) * 8u;  // 8 resident 256-item groups per CU
+    b->scratch_error = resolve_scratch(dev, &b->scratch);   // reported by lbm_batch_residuals_noalloc
 #undef CREATE_TRY
     if (int rc = build_tiles(b))
         return cleanup(rc);
@@ -469,6 +540,10 @@ int lbm_batch_init(lbm_batch *b)
     std::vector<uint16_t> codes(b->code_nodes);
     BATCH_TRY(hipMemcpy(codes.data(), b->codes, codes.size() * sizeof(uint16_t), hipMemcpyDeviceToHost));
     b->fluid_nodes = uint64_t(std::count_if(codes.begin(), codes.end(), [](uint16_t c) { return c & LBM_CODE_FLUID; }));
+    // the residual snapshot (2 doubles per node place) is allocated here, at
+    // set-up, so that lbm_batch_residuals_noalloc never allocates
+    if (!b->snap)
+        BATCH_TRY(hipMalloc(&b->snap, 2ull * b->code_nodes * sizeof(double)));
     b->iteration = 0;
     b->initialized = true;
     b->snap_valid = false;
@@ -685,6 +760,61 @@ int lbm_batch_residuals(lbm_batch *b, double *out, uint32_t count)
         else
             out[d] = den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? 1.0 : 0.0);
     }
+    return LBM_OK;
+}
+
+int lbm_batch_residuals_noalloc(lbm_batch *b, double *out, uint32_t count)
+{
+    if (!b || !out || count != b->doms.size())
+        return lbm_set_error(LBM_ERR_INVALID_ARGUMENT,
+                             "lbm_batch_residuals_noalloc: count must equal the domain count");
+    if (!b->initialized)
+        return lbm_set_error(LBM_ERR_STATE, "lbm_batch_residuals_noalloc: call lbm_batch_init first");
+    if (b->ntiles > kScratchTiles || count > kScratchDomains)
+        return lbm_set_error(LBM_ERR_INVALID_ARGUMENT,
+                             "lbm_batch_residuals_noalloc: " + std::to_string(b->ntiles) + " tiles and " +
+                                 std::to_string(count) + " domains exceed the static scratch (at most " +
+                                 std::to_string(kScratchTiles) + " tiles and " + std::to_string(kScratchDomains) +
+                                 " domains); use lbm_batch_residuals");
+    if (!b->scratch)
+        return lbm_set_error(LBM_ERR_DEVICE, std::string("lbm_batch_residuals_noalloc: hipGetSymbolAddress: ") +
+                                                 hipGetErrorString(b->scratch_error));
+    if (!b->snap)
+        return lbm_set_error(LBM_ERR_STATE, "lbm_batch_residuals_noalloc: no residual snapshot (lbm_batch_init "
+                                            "allocates it)");
+    if (!b->snap_valid)
+    {
+        BATCH_TRY(hipMemsetAsync(b->snap, 0, 2ull * b->code_nodes * sizeof(double), b->stream));
+        b->snap_valid = true;
+    }
+    double *partials = b->scratch;
+    double *residuals = b->scratch + 2ull * kScratchTiles;
+    const void *src = b->arena[b->iteration & 1u];
+    switch (b->storage)
+    {
+    case LBM_STORE_F64:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_F64>), dim3(b->ntiles), dim3(kTileThreads), 0, b->stream,
+                           b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    case LBM_STORE_F32:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_F32>), dim3(b->ntiles), dim3(kTileThreads), 0, b->stream,
+                           b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    case LBM_STORE_F16_SHIFTED:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_F16_SHIFTED>), dim3(b->ntiles), dim3(kTileThreads), 0,
+                           b->stream, b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    default:
+        hipLaunchKernelGGL((k_batch_residual<LBM_STORE_BF16_SHIFTED>), dim3(b->ntiles), dim3(kTileThreads), 0,
+                           b->stream, b->d_doms, b->d_tiles, src, b->codes, b->snap, partials);
+        break;
+    }
+    BATCH_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_residual_domains, dim3((count + 255u) / 256u), dim3(256), 0, b->stream, b->d_tiles,
+                       b->ntiles, count, partials, residuals);
+    BATCH_TRY(hipGetLastError());
+    BATCH_TRY(hipMemcpyAsync(out, residuals, count * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    BATCH_TRY(hipStreamSynchronize(b->stream));
     return LBM_OK;
 }
 
