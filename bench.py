@@ -19,7 +19,7 @@ before the line is printed:
                          algorithm x layout for about S seconds)
   --precision-study OUT  FP32/FP16/BF16 storage against FP64 after a fixed
                          number of steps (--study-scenario, --study-taus,
-                         --study-steps)
+                         --study-steps, --study-average)
   --steady-study OUT     the same at the steady state of a 1024x256 channel
   --batched OUT          512 uneven channels: per-domain launches, one
                          persistent launch per step, autotuned graph replay
@@ -149,14 +149,21 @@ def sweep(out_path: str, steps: int, warmup: int, cpu_budget_s: float = 0.0) -> 
                                           "rows": rows}, indent=1))
 
 
-def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: str = "cylinder") -> None:
+def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: str = "cylinder",
+                    average: int = 0, sample_every: int = 100) -> None:
     """Reduced-precision storage against FP64 on the 8192 x 2048 cylinder
     channel (disc r = 64 at x = 1024, centred), inlet 1.02 / outlet 1.0, or
-    (scenario "porous") on the steady porous channel of the default workload.
+    (scenario "porous") on the porous channel of the default workload.
     Errors are over fluid nodes, relative to the FP64 run of the same tau:
     L2 = ||a - a64||_2 / ||a64||_2 and Linf = max|a - a64| / max|a64| for the
     density and the momentum vector j; mass drift = sum rho(T) / sum rho(0) - 1
-    (FP64 sums on the GPU); drag = momentum-exchange force on the disc."""
+    (FP64 sums of the populations on the GPU); drag = momentum-exchange force
+    on the solids.  With average > 0 the fields and the drag are averaged over
+    the last `average` steps (a sample every `sample_every` steps): the
+    cylinder's wake sheds vortices, and instantaneous fields of two runs drift
+    out of phase.  A run is finite when the FP64 sums of its fluid
+    populations are (the reported fields clamp NaN to 0); a tau whose FP64 run
+    is not finite is reported as FP64-unstable, without error figures."""
     rows = []
     for tau in taus:
         ref = None
@@ -168,25 +175,39 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: 
             else:
                 doc = workloads.cylinder_channel(tau, storage=storage, steps=steps)
             doc["mi355x"]["rounding"] = rounding or "nearest"
-            lat = pa.Lattice.from_settings(doc)
-            try:
-                fluid_mask = None
+            with pa.Lattice.from_settings(doc) as lat:
                 m0 = lat.reduce()["mass"]
-                ms = lat.run(steps, 0)
+                window = min(average, steps)
+                ms = lat.run(steps - window, 0) if steps > window else 0.0
+                acc, fx, fy, samples = None, 0.0, 0.0, 0
+                if window:
+                    done = 0
+                    while done < window:
+                        k = min(sample_every, window - done)
+                        ms += lat.run(k, 0)
+                        done += k
+                        fields = np.stack(lat.macros()[:3])
+                        red = lat.reduce()
+                        acc = fields if acc is None else acc + fields
+                        fx, fy, samples = fx + red["fx"], fy + red["fy"], samples + 1
+                    rho, jx, jy = acc / samples
+                    fx, fy = fx / samples, fy / samples
                 red = lat.reduce()
-                rho, jx, jy, _ = lat.macros()
-            finally:
-                lat.close()
+                if not window:
+                    rho, jx, jy, _ = lat.macros()
+                    fx, fy = red["fx"], red["fy"]
+            finite = bool(np.isfinite([red["mass"], red["jx"], red["jy"]]).all())
             fluid = rho > -1
             row = {"tau": tau, "storage": storage, "rounding": rounding or "nearest", "steps": steps,
-                   "fluid_nodes": int(fluid.sum()),
+                   "averaged_steps": window, "fluid_nodes": int(fluid.sum()),
                    "mflups": round(int(fluid.sum()) * steps / (ms * 1e-3) / 1e6, 1),
-                   "mass_drift": red["mass"] / m0 - 1.0, "drag_x": red["fx"], "drag_y": red["fy"],
-                   "finite": bool(np.isfinite(rho[fluid]).all() and np.isfinite(jx[fluid]).all())}
+                   "mass_drift": red["mass"] / m0 - 1.0, "drag_x": fx, "drag_y": fy, "finite": finite}
             if storage == "f64":
-                ref = (rho, jx, jy, red)
-            else:
-                r64, jx64, jy64, red64 = ref
+                ref = (rho, jx, jy, fx) if finite else None
+                if not finite:
+                    row["fp64_unstable"] = True
+            elif ref is not None and finite:
+                r64, jx64, jy64, fx64 = ref
                 f = fluid & (r64 > -1)
                 dj = np.hypot(jx[f] - jx64[f], jy[f] - jy64[f])
                 j64 = np.hypot(jx64[f], jy64[f])
@@ -195,14 +216,15 @@ def precision_study(out_path: str, steps: int, taus=(0.52, 0.6, 0.8), scenario: 
                     "rho_linf": float(np.abs(rho[f] - r64[f]).max() / np.abs(r64[f]).max()),
                     "u_l2": float(np.linalg.norm(dj) / np.linalg.norm(j64)),
                     "u_linf": float(dj.max() / j64.max()),
-                    "drag_x_rel": float(abs(red["fx"] - red64["fx"]) / abs(red64["fx"])),
+                    "drag_x_rel": float(abs(fx - fx64) / abs(fx64)),
                 })
             rows.append(row)
             print(json.dumps(row), file=sys.stderr, flush=True)
     workload = (f"porous-circles {INNER_H}x{INNER_V} (seed 20261202, radii U[4,16), porosity 0.7), "
                 "rho_in 1.05 / rho_out 1.0" if scenario == "porous" else
                 f"cylinder r=64 at x=1024 in {INNER_H}x{INNER_V}, rho_in 1.02 / rho_out 1.0")
-    Path(out_path).write_text(json.dumps({"workload": workload, "steps": steps, "rows": rows}, indent=1))
+    Path(out_path).write_text(json.dumps({"workload": workload, "steps": steps, "averaged_steps": average,
+                                          "rows": rows}, indent=1))
 
 
 def steady_precision_study(out_path: str, inner_h: int = 1024, inner_v: int = 256, taus=(0.6, 0.8),
@@ -311,6 +333,8 @@ def main() -> int:
     ap.add_argument("--study-scenario", default="cylinder", choices=("cylinder", "porous"))
     ap.add_argument("--study-taus", default="0.52,0.6,0.8")
     ap.add_argument("--study-steps", type=int, default=20000)
+    ap.add_argument("--study-average", type=int, default=None,
+                    help="average fields and drag over the last N steps (default: 5000 for the cylinder, 0 for porous)")
     ap.add_argument("--steady-study", metavar="OUT.json",
                     help="reduced-precision storage against FP64 at the steady state of a 1024x256 porous channel")
     ap.add_argument("--batched", metavar="OUT.json")
@@ -326,7 +350,10 @@ def main() -> int:
         sweep(args.sweep, min(args.steps, 500), min(args.warmup, 20), args.sweep_cpu_seconds)
     if args.precision_study:
         taus = tuple(float(t) for t in args.study_taus.split(","))
-        precision_study(args.precision_study, args.study_steps, taus, args.study_scenario)
+        average = args.study_average
+        if average is None:
+            average = 5000 if args.study_scenario == "cylinder" else 0
+        precision_study(args.precision_study, args.study_steps, taus, args.study_scenario, average)
     if args.steady_study:
         steady_precision_study(args.steady_study)
     if args.batched:
