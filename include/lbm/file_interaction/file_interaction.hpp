@@ -1,5 +1,5 @@
 // Settings file <-> Properties.  Same functions, keys and semantics as the
-// reference (include/lbm/file_interaction/file_interaction.hpp:98-118,
+// reference (include/lbm/file_interaction/file_interaction.hpp:30-49,
 // src/lbm/file_interaction/file_interaction.cpp:11-69):
 //   algorithmic.{algorithm, dataLayout, debugMode, frameUpdateInterval,
 //                timeSteps, workGroupSize}

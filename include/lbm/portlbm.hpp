@@ -1,4 +1,4 @@
-// Umbrella header of the library (reference: include/lbm/portlbm.hpp:127-157).
+// Umbrella header of the library (reference: include/lbm/portlbm.hpp:29-53).
 #ifndef PORTLBM_HPP
 #define PORTLBM_HPP
 

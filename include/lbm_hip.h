@@ -32,7 +32,7 @@
  *     include/lbm/core/simulation.hpp:240-254              lbm_set_populations
  *   Data::phase_information                              lbm_get_phase
  *   get_neighbor + EmplaceBounceBackKernel addressing    lbm_get_node_codes
- *     include/lbm/core/access.hpp:203-213,
+ *     include/lbm/core/access.hpp:142-152,
  *     include/lbm/gpu/nptl/nptl_kernels.hpp:471-491
  *   queue->wait()                                        lbm_sync
  *   exceptions crossing the API                          int status + lbm_last_error
@@ -63,11 +63,15 @@ extern "C" {
 /* ---- enumerations ------------------------------------------------------ */
 /* core/constants.hpp:43-48 */
 enum lbm_algorithm { LBM_ALG_NPTL = 0, LBM_ALG_LPTL = 1, LBM_ALG_NPOL = 2, LBM_ALG_NSOL = 3 };
-/* core/constants.hpp:50, core/access.hpp:102-172 */
+/* core/constants.hpp:50, core/access.hpp:41-110 */
 enum lbm_layout { LBM_LAYOUT_COLLISION = 0, LBM_LAYOUT_STREAM = 1, LBM_LAYOUT_BUNDLE = 2 };
 /* Population storage type.  The reference has only the compile-time
- * real_type switch (core/constants.hpp:17-21, double or float).  F16/BF16
- * store the deviation f_i - w_i; collision arithmetic then runs in FP32. */
+ * real_type switch (core/constants.hpp:17-21, double or float); F32 computes
+ * as its float build does.  F16/BF16 store the deviation f_i - w_i rho_ref(x)
+ * from the rest state at a reference density of the population's column
+ * (the mean density of the column's fluid nodes, refreshed every 128 steps;
+ * rho_in and rho_out on the ghost columns; 1 for batched domains); collision
+ * arithmetic then runs in FP32. */
 enum lbm_storage {
     LBM_STORE_F64 = 0,
     LBM_STORE_F32 = 1,
@@ -191,7 +195,7 @@ int lbm_get_iteration(lbm_ctx *ctx, uint64_t *iteration);
 int lbm_last_run_ms(lbm_ctx *ctx, double *ms);
 
 /* Copy the four result arrays, each (H-2)*(V-2) values in row-major inner
- * order r = (x-1) + (y-1)*(H-2) (core/access.hpp:269-273).  Any pointer may
+ * order r = (x-1) + (y-1)*(H-2) (core/access.hpp:208-212).  Any pointer may
  * be NULL to skip that array. */
 int lbm_get_macros(lbm_ctx *ctx, double *density, double *x_velocity, double *y_velocity,
                    double *absolute_velocity, size_t count);
@@ -213,8 +217,11 @@ int lbm_get_frame(lbm_ctx *ctx, double *density, double *x_velocity, double *y_v
                   double *absolute_velocity, size_t count, uint64_t *iteration);
 
 /* Current source lattice: 9 * expanded node count values in the context's
- * layout order (core/access.hpp:117-171), widened to double; shifted
- * storage is returned as f_i (the weight added back).  Fluid and ghost nodes
+ * layout order (core/access.hpp:56-60, 81-85, 106-110), widened to double;
+ * shifted storage is returned as f_i (w_i times the column's reference
+ * density added back).  The conversion runs in chunks through a staging
+ * buffer of at most 32 MiB that the context keeps after the first call
+ * (freed by lbm_destroy).  Fluid and ghost nodes
  * are current.  Solid nodes' populations are never read (half-way
  * bounce-back takes the fluid node's own), and the fused kernels leave them
  * zero or as they were.  npol/nsol with tiles at most 64 nodes wide keep

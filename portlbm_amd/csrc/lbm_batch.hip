@@ -45,6 +45,19 @@ struct BatchDom
     double omega, rho_out;
 };
 
+// Set-up geometry of one domain: an unexpanded lattice with its own pitch.
+// The batched 16-bit forms store f_d - w_d (rho_ref = 1 in every column: `ones`).
+inline Geom batch_geom(const BatchDom &dm, const float *ones)
+{
+    Geom g{dm.H, dm.H * dm.V, dm.P, kBatchX0, dm.S};
+    g.ref = ones;
+    g.H = dm.H;
+    g.buffered = 0u;
+    g.sub_h = 1u;
+    g.ghost_xe = dm.H - 1u;
+    return g;
+}
+
 struct BatchTile
 {
     uint32_t dom;
@@ -64,6 +77,7 @@ struct BatchArgs
     int parity;
     uint32_t write_macros;
     uint32_t nan_protection;
+    const float *ones;  // rho_ref of the 16-bit forms: 1 in every column
 };
 
 template <int S>
@@ -100,11 +114,11 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_step(BatchArgs a)
             p.H = dm.H;
             p.write_macros = a.write_macros;
             p.nan_protection = a.nan_protection;
-            pull<LBM_LAYOUT_STREAM, S>(src + dm.lat, pn, code, dm.P, dm.S, f);
-            finish_node<LBM_LAYOUT_STREAM, S>(p, code, x, y, f);
+            pull<LBM_LAYOUT_STREAM, S>(src + dm.lat, pn, code, dm.P, dm.S, f, rt3_of<S>(a.ones, x));
+            finish_node<LBM_LAYOUT_STREAM, S>(p, code, x, y, f, 1.f);
         }
         if (fluid || fill)
-            store_node<LBM_LAYOUT_STREAM, S>(dst + dm.lat, pn, dm.S, f);
+            store_node<LBM_LAYOUT_STREAM, S>(dst + dm.lat, pn, dm.S, f, 0u, 0.f);  // rho_ref = 1 throughout
     }
 }
 
@@ -131,7 +145,8 @@ __global__ __launch_bounds__(kTileThreads) void k_batch_residual(const BatchDom 
 #pragma unroll
             for (int d = 0; d < 9; ++d)
             {
-                const double f = Storage<S>::decode(src[dm.lat + at<LBM_LAYOUT_STREAM>(pn, uint32_t(d), dm.S)], d);
+                const double f = Storage<S>::decode(src[dm.lat + at<LBM_LAYOUT_STREAM>(pn, uint32_t(d), dm.S)], d,
+                                                    1.f);  // rho_ref = 1 in the batch
                 jx += f * double(cx_of(d));
                 jy += f * double(cy_of(d));
             }
@@ -268,6 +283,7 @@ struct lbm_batch
     double *snap = nullptr;      // 2 doubles per node place (residual snapshot, allocated by lbm_batch_init)
     bool snap_valid = false;
     double *scratch = nullptr;   // device address of this GPU's `scratch` (null if the lookup failed)
+    float *ones = nullptr;       // rho_ref of the 16-bit forms: max H + 64 ones
     hipError_t scratch_error = hipSuccess;
 };
 
@@ -314,6 +330,7 @@ template <int S>
 int launch_step(lbm_batch *b, int parity, bool macros, const BatchTile *tiles, uint32_t ntiles, uint32_t grid)
 {
     BatchArgs a{};
+    a.ones = b->ones;
     a.doms = b->d_doms;
     a.tiles = tiles;
     a.ntiles = ntiles;
@@ -356,7 +373,7 @@ int init_domain_lattice(lbm_batch *b, size_t d, typename Storage<S>::T *l0)
 {
     const BatchDom &dm = b->doms[d];
     const lbm_batch_domain &ds = b->desc[d];
-    const Geom g{dm.H, dm.H * dm.V, dm.P, kBatchX0, dm.S};
+    const Geom g = batch_geom(dm, b->ones);
     hipLaunchKernelGGL((k_init_rest<LBM_LAYOUT_STREAM, S>), dim3((g.N + 255u) / 256u), dim3(256), 0, b->stream,
                        l0 + dm.lat, g);
     InoutValues v;
@@ -415,6 +432,7 @@ void lbm_batch_destroy(lbm_batch *b)
     (void)hipFree(b->d_doms);
     (void)hipFree(b->d_tiles);
     (void)hipFree(b->snap);
+    (void)hipFree(b->ones);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -498,6 +516,14 @@ int lbm_batch_create(const lbm_batch_domain *domains, uint32_t count, int32_t st
     CREATE_TRY(hipMalloc(&b->res, b->res_doubles * sizeof(double)));
     CREATE_TRY(hipMalloc(&b->uy, b->uy_doubles * sizeof(double)));
     CREATE_TRY(hipMalloc(&b->d_doms, count * sizeof(BatchDom)));
+    {
+        uint32_t hmax = 0;
+        for (const BatchDom &dm : b->doms)
+            hmax = std::max(hmax, dm.H);
+        const std::vector<float> ones(hmax + 64u, 1.f);
+        CREATE_TRY(hipMalloc(&b->ones, ones.size() * sizeof(float)));
+        CREATE_TRY(hipMemcpy(b->ones, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     int dev = 0, cus = 256;
     CREATE_TRY(hipGetDevice(&dev));
     CREATE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -859,7 +885,7 @@ int lbm_batch_set_equilibrium(lbm_batch *b, uint32_t domain, const double *rho, 
     const double *src[3] = {rho, jx, jy};
     for (int k = 0; k < 3 && e == hipSuccess; ++k)
         e = hipMemcpyAsync(tmp + k * count, src[k], count * sizeof(double), hipMemcpyHostToDevice, b->stream);
-    const Geom g{dm.H, dm.H * dm.V, dm.P, kBatchX0, dm.S};
+    const Geom g = batch_geom(dm, b->ones);
     void *lat = b->arena[b->iteration & 1u];
     if (e == hipSuccess)
     {

@@ -57,6 +57,25 @@ __device__ __forceinline__ uint32_t at(uint32_t node, uint32_t d, uint32_t total
 // populations leaving them out changes no result; once a population is
 // infinite, 0 * inf is NaN in the reference, and the full sums propagate a
 // blow-up exactly as it does (FP64, FP32).  The 16-bit forms leave them out.
+//
+// The 16-bit forms store the deviation f_d - w_d rho_ref(x) of a population
+// from the rest equilibrium at a reference density of its column: x is the
+// unexpanded column of the node that holds the population, and rho_ref(x)
+// the mean density of the column's fluid nodes, refreshed every few hundred
+// steps (a table of H floats; the ghost columns keep rho_in and rho_out).
+// The stored deviation then carries the velocity and the density's departure
+// from the column mean instead of the pressure drop along the channel, and
+// the 16-bit grid is that much finer where the velocity signal lives.  A
+// fixed reference of 1 (the shift w_d alone) leaves the density drop in the
+// stored value; a fixed linear drop from rho_in to rho_out misrepresents the
+// fluid still at rest while the pressure wave crosses the channel.  Measured
+// in FP16 on a 1024 x 256 porous channel against FP64 (momentum L2; rounding
+// to nearest / stochastic): after 500 steps w_d 5.3e-4 / 7.4e-4, linear
+// 1.1e-3 / 1.4e-3, column mean 3.1e-4 / 4.2e-4; after 5000 steps w_d
+// 3.1e-3 / 1.6e-3, linear 1.1e-3 / 6.3e-4, column mean 1.0e-3 / 5.1e-4.
+// `rt` below is rho_ref of the node whose population is read or written;
+// FP64 and FP32 ignore it.
+
 template <int S>
 struct Storage;
 
@@ -67,11 +86,14 @@ struct Storage<LBM_STORE_F64>
     using C = double;
     static constexpr bool kPromote = false;
     static constexpr bool kFull = true;
-    static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
-    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
-    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
-    static __device__ __forceinline__ T encode(double v, int) { return v; }
-    static __device__ __forceinline__ double decode(T v, int) { return v; }
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int, float = 1.f) { return p[i]; }
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int, float = 1.f) { p[i] = v; }
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int, float = 1.f)
+    {
+        __builtin_nontemporal_store(v, p + i);
+    }
+    static __device__ __forceinline__ T encode(double v, int, float = 1.f) { return v; }
+    static __device__ __forceinline__ double decode(T v, int, float = 1.f) { return v; }
 };
 
 template <>
@@ -81,11 +103,14 @@ struct Storage<LBM_STORE_F32>
     using C = float;
     static constexpr bool kPromote = true;
     static constexpr bool kFull = true;
-    static __device__ __forceinline__ C load(const T *p, uint32_t i, int) { return p[i]; }
-    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int) { p[i] = v; }
-    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int) { __builtin_nontemporal_store(v, p + i); }
-    static __device__ __forceinline__ T encode(double v, int) { return float(v); }
-    static __device__ __forceinline__ double decode(T v, int) { return double(v); }
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int, float = 1.f) { return p[i]; }
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int, float = 1.f) { p[i] = v; }
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int, float = 1.f)
+    {
+        __builtin_nontemporal_store(v, p + i);
+    }
+    static __device__ __forceinline__ T encode(double v, int, float = 1.f) { return float(v); }
+    static __device__ __forceinline__ double decode(T v, int, float = 1.f) { return double(v); }
 };
 
 // 32-bit integer hash (xor-shift-multiply finaliser); random bits for
@@ -113,8 +138,13 @@ __device__ __forceinline__ _Float16 f32_to_f16_sr(float v, uint32_t r)
     return _Float16(__uint_as_float(u));
 }
 
-// FP16 deviation storage: HBM holds f_i - w_i as IEEE binary16 (RNE, or
-// stochastic through store_sr).
+// The shift of direction d at a node whose rho_ref is rt (float).
+__device__ __forceinline__ float shift_of(int d, float rt) { return weight<float>(d) * rt; }
+
+// FP16 deviation storage: HBM holds f_d - w_d rho_ref(x) as IEEE binary16
+// (RNE, or stochastic through store_sr).  decode gives the stored value
+// exactly (binary16 and the float shift are exact in double); a kernel's
+// load rounds that sum to float.
 template <>
 struct Storage<LBM_STORE_F16_SHIFTED>
 {
@@ -122,28 +152,34 @@ struct Storage<LBM_STORE_F16_SHIFTED>
     using C = float;
     static constexpr bool kPromote = false;
     static constexpr bool kFull = false;
-    static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int d, float rt)
     {
-        return float(p[i]) + weight<float>(d);
+        return float(p[i]) + shift_of(d, rt);
     }
-    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int d)
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int d, float rt)
     {
-        p[i] = _Float16(v - weight<float>(d));
+        p[i] = _Float16(v - shift_of(d, rt));
     }
-    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int d)
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int d, float rt)
     {
-        __builtin_nontemporal_store(_Float16(v - weight<float>(d)), p + i);
+        __builtin_nontemporal_store(_Float16(v - shift_of(d, rt)), p + i);
     }
-    static __device__ __forceinline__ void store_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    static __device__ __forceinline__ void store_sr(T *p, uint32_t i, C v, int d, float rt, uint32_t r)
     {
-        p[i] = f32_to_f16_sr(v - weight<float>(d), r);
+        p[i] = f32_to_f16_sr(v - shift_of(d, rt), r);
     }
-    static __device__ __forceinline__ void store_nt_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    static __device__ __forceinline__ void store_nt_sr(T *p, uint32_t i, C v, int d, float rt, uint32_t r)
     {
-        __builtin_nontemporal_store(f32_to_f16_sr(v - weight<float>(d), r), p + i);
+        __builtin_nontemporal_store(f32_to_f16_sr(v - shift_of(d, rt), r), p + i);
     }
-    static __device__ __forceinline__ T encode(double v, int d) { return _Float16(float(v - weight<double>(d))); }
-    static __device__ __forceinline__ double decode(T v, int d) { return double(float(v)) + weight<double>(d); }
+    static __device__ __forceinline__ T encode(double v, int d, float rt)
+    {
+        return _Float16(float(v - double(shift_of(d, rt))));
+    }
+    static __device__ __forceinline__ double decode(T v, int d, float rt)
+    {
+        return double(float(v)) + double(shift_of(d, rt));
+    }
 };
 
 // bfloat16 bit conversions with round-to-nearest-even.
@@ -167,7 +203,8 @@ __device__ __forceinline__ uint16_t f32_to_bf16_sr(float f, uint32_t r)
     return uint16_t((u + r) >> 16);
 }
 
-// BF16 deviation storage: HBM holds f_i - w_i as bfloat16 (RNE).
+// BF16 deviation storage: HBM holds f_d - w_d rho_ref(x) as bfloat16 (RNE,
+// or stochastic through store_sr).
 template <>
 struct Storage<LBM_STORE_BF16_SHIFTED>
 {
@@ -175,28 +212,34 @@ struct Storage<LBM_STORE_BF16_SHIFTED>
     using C = float;
     static constexpr bool kPromote = false;
     static constexpr bool kFull = false;
-    static __device__ __forceinline__ C load(const T *p, uint32_t i, int d)
+    static __device__ __forceinline__ C load(const T *p, uint32_t i, int d, float rt)
     {
-        return bf16_to_f32(p[i]) + weight<float>(d);
+        return bf16_to_f32(p[i]) + shift_of(d, rt);
     }
-    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int d)
+    static __device__ __forceinline__ void store(T *p, uint32_t i, C v, int d, float rt)
     {
-        p[i] = f32_to_bf16(v - weight<float>(d));
+        p[i] = f32_to_bf16(v - shift_of(d, rt));
     }
-    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int d)
+    static __device__ __forceinline__ void store_nt(T *p, uint32_t i, C v, int d, float rt)
     {
-        __builtin_nontemporal_store(f32_to_bf16(v - weight<float>(d)), p + i);
+        __builtin_nontemporal_store(f32_to_bf16(v - shift_of(d, rt)), p + i);
     }
-    static __device__ __forceinline__ void store_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    static __device__ __forceinline__ void store_sr(T *p, uint32_t i, C v, int d, float rt, uint32_t r)
     {
-        p[i] = f32_to_bf16_sr(v - weight<float>(d), r);
+        p[i] = f32_to_bf16_sr(v - shift_of(d, rt), r);
     }
-    static __device__ __forceinline__ void store_nt_sr(T *p, uint32_t i, C v, int d, uint32_t r)
+    static __device__ __forceinline__ void store_nt_sr(T *p, uint32_t i, C v, int d, float rt, uint32_t r)
     {
-        __builtin_nontemporal_store(f32_to_bf16_sr(v - weight<float>(d), r), p + i);
+        __builtin_nontemporal_store(f32_to_bf16_sr(v - shift_of(d, rt), r), p + i);
     }
-    static __device__ __forceinline__ T encode(double v, int d) { return f32_to_bf16(float(v - weight<double>(d))); }
-    static __device__ __forceinline__ double decode(T v, int d) { return double(bf16_to_f32(v)) + weight<double>(d); }
+    static __device__ __forceinline__ T encode(double v, int d, float rt)
+    {
+        return f32_to_bf16(float(v - double(shift_of(d, rt))));
+    }
+    static __device__ __forceinline__ double decode(T v, int d, float rt)
+    {
+        return double(bf16_to_f32(v)) + double(shift_of(d, rt));
+    }
 };
 
 // ---- physical placement of the lattice in HBM ----------------------------
@@ -245,7 +288,52 @@ struct StepParams
     // read this step and written for the next
     uint32_t side_rows_in, side_cols_in, side_rows_out, side_cols_out;
     uint32_t cnt_h, cnt_v;  // tiles per row / column
+    // rho_ref per unexpanded column (16-bit forms): the source lattice was
+    // stored with ref_in, this step stores with ref_out (the same table
+    // except on a step that changes the reference)
+    const float *ref_in;
+    const float *ref_out;
 };
+
+// Is S one of the 16-bit forms (whose stored values depend on rho_ref)?
+template <int S>
+constexpr bool shifted_storage()
+{
+    return S == LBM_STORE_F16_SHIFTED || S == LBM_STORE_BF16_SHIFTED;
+}
+
+// rho_ref of unexpanded column xu for the 16-bit forms (1 otherwise, unused).
+template <int S>
+__device__ __forceinline__ float rt_of(const float *ref, uint32_t xu)
+{
+    if constexpr (shifted_storage<S>())
+        return ref[xu];
+    else
+        return 1.f;
+}
+
+// rho_ref of columns xu - 1, xu, xu + 1 (scalars and a select, so that no
+// private array reaches scratch memory).  The 16-bit forms collide in
+// deviation form: a node works with g_d = f_d - w_d rho_ref(xu) of its own
+// column, so a population pulled from column xu - cx is decoded against the
+// difference rel(cx) = rho_ref(xu - cx) - rho_ref(xu) (exact: both are floats
+// near 1), and its own slots against 0.
+struct Rt3
+{
+    float l, c, r;
+    __device__ __forceinline__ float own() const { return c; }
+    __device__ __forceinline__ float src(int cx) const { return cx > 0 ? l : (cx < 0 ? r : c); }
+    __device__ __forceinline__ float rel(int cx) const { return src(cx) - c; }
+};
+
+template <int S>
+__device__ __forceinline__ Rt3 rt3_of(const float *ref, uint32_t xu)
+{
+    if constexpr (shifted_storage<S>())
+        return Rt3{ref[xu ? xu - 1u : 0u], ref[xu], ref[xu + 1u]};  // xu = 0: the inlet ghost, which pulls nothing
+    else
+        return Rt3{1.f, 1.f, 1.f};
+}
 
 // NaN clamp of WITH_NAN_PROTECTION (nptl_kernels.hpp:413-432): NaN or values
 // above FLT_MAX become 0.  Applied to outputs only.
@@ -261,9 +349,14 @@ __device__ __forceinline__ C clamp_output(C v, uint32_t on)
 // Pull 9 populations.  Bit d of `code` set: the upstream node n - c_d is solid
 // and half-way bounce-back supplies this node's own population 8-d.
 // p is the node's physical index, P the row pitch and Sd the direction stride.
+// r3: rho_ref around the node's unexpanded column xu (rt3_of): the population
+// pulled for direction d is held by column xu - c_x(d) (buffer columns are
+// transparent in unexpanded coordinates), or by the node itself when it
+// bounces back.  The 16-bit forms return deviations from the node's own
+// reference (Rt3::rel); FP64 and FP32 return the populations.
 template <int L, int S>
 __device__ __forceinline__ void pull(const typename Storage<S>::T *src, uint32_t p, uint32_t code,
-                                     uint32_t P, uint32_t Sd, typename Storage<S>::C f[9])
+                                     uint32_t P, uint32_t Sd, typename Storage<S>::C f[9], const Rt3 &r3)
 {
 #pragma unroll
     for (int d = 0; d < 9; ++d)
@@ -271,8 +364,20 @@ __device__ __forceinline__ void pull(const typename Storage<S>::T *src, uint32_t
         const bool bb = (code >> d) & 1u;
         const uint32_t m = bb ? p : uint32_t(int32_t(p) - cy_of(d) * int32_t(P) - cx_of(d));
         const int dd = bb ? 8 - d : d;
-        f[d] = Storage<S>::load(src, at<L>(m, uint32_t(dd), Sd), dd);
+        f[d] = Storage<S>::load(src, at<L>(m, uint32_t(dd), Sd), dd, bb ? 0.f : r3.rel(cx_of(d)));
     }
+}
+
+// A population decoded against 0 by the lane that holds it, moved to a lane
+// whose reference differs by rel (16-bit forms; the same operations as
+// decoding it there).
+template <int S>
+__device__ __forceinline__ typename Storage<S>::C rebase(typename Storage<S>::C v, int d, float rel)
+{
+    if constexpr (shifted_storage<S>())
+        return v + shift_of(d, rel);
+    else
+        return v;
 }
 
 // Zou-He pressure outlet at x = H-2 (non_buffered.hpp:95-129).  The three
@@ -372,11 +477,12 @@ __device__ __forceinline__ void collide(C f[9], C omega, C &rho, C &jx, C &jy, C
 // (FP16/BF16 only).  The random bits of place p: h = mix32(sr ^ p), then per
 // direction one step of a 32-bit LCG (PCG's multiplier and increment), of
 // which the top 13 (FP16) or 16 (BF16) bits are used.
+// rt: rho_ref of the node (16-bit forms).
 template <int L, int S>
 __device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t p, uint32_t Sd,
-                                           const typename Storage<S>::C f[9], uint32_t sr = 0)
+                                           const typename Storage<S>::C f[9], uint32_t sr = 0, float rt = 1.f)
 {
-    if constexpr (S == LBM_STORE_F16_SHIFTED || S == LBM_STORE_BF16_SHIFTED)
+    if constexpr (shifted_storage<S>())
     {
         if (sr)
         {
@@ -388,9 +494,9 @@ __device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t
                 h = h * 747796405u + 2891336453u;
                 const uint32_t r = h >> drop;
                 if constexpr (L == LBM_LAYOUT_STREAM)
-                    Storage<S>::store_nt_sr(dst, at<L>(p, uint32_t(d), Sd), f[d], d, r);
+                    Storage<S>::store_nt_sr(dst, at<L>(p, uint32_t(d), Sd), f[d], d, rt, r);
                 else
-                    Storage<S>::store_sr(dst, at<L>(p, uint32_t(d), Sd), f[d], d, r);
+                    Storage<S>::store_sr(dst, at<L>(p, uint32_t(d), Sd), f[d], d, rt, r);
             }
             return;
         }
@@ -399,9 +505,9 @@ __device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t
     for (int d = 0; d < 9; ++d)
     {
         if constexpr (L == LBM_LAYOUT_STREAM)
-            Storage<S>::store_nt(dst, at<L>(p, uint32_t(d), Sd), f[d], d);
+            Storage<S>::store_nt(dst, at<L>(p, uint32_t(d), Sd), f[d], d, rt);
         else
-            Storage<S>::store(dst, at<L>(p, uint32_t(d), Sd), f[d], d);
+            Storage<S>::store(dst, at<L>(p, uint32_t(d), Sd), f[d], d, rt);
     }
 }
 
@@ -409,9 +515,9 @@ __device__ __forceinline__ void store_node(typename Storage<S>::T *dst, uint32_t
 // same random stream), for a caller that also writes them elsewhere.
 template <int S>
 __device__ __forceinline__ void node_bits(const typename Storage<S>::C f[9], uint32_t p, uint32_t sr,
-                                          typename Storage<S>::T t[9])
+                                          typename Storage<S>::T t[9], float rt = 1.f)
 {
-    if constexpr (S == LBM_STORE_F16_SHIFTED || S == LBM_STORE_BF16_SHIFTED)
+    if constexpr (shifted_storage<S>())
     {
         if (sr)
         {
@@ -423,9 +529,9 @@ __device__ __forceinline__ void node_bits(const typename Storage<S>::C f[9], uin
                 h = h * 747796405u + 2891336453u;
                 const uint32_t r = h >> drop;
                 if constexpr (S == LBM_STORE_F16_SHIFTED)
-                    t[d] = f32_to_f16_sr(f[d] - weight<float>(d), r);
+                    t[d] = f32_to_f16_sr(f[d] - shift_of(d, rt), r);
                 else
-                    t[d] = f32_to_bf16_sr(f[d] - weight<float>(d), r);
+                    t[d] = f32_to_bf16_sr(f[d] - shift_of(d, rt), r);
             }
             return;
         }
@@ -433,9 +539,9 @@ __device__ __forceinline__ void node_bits(const typename Storage<S>::C f[9], uin
         for (int d = 0; d < 9; ++d)
         {
             if constexpr (S == LBM_STORE_F16_SHIFTED)
-                t[d] = _Float16(f[d] - weight<float>(d));
+                t[d] = _Float16(f[d] - shift_of(d, rt));
             else
-                t[d] = f32_to_bf16(f[d] - weight<float>(d));
+                t[d] = f32_to_bf16(f[d] - shift_of(d, rt));
         }
     }
     else
@@ -469,8 +575,9 @@ __device__ __forceinline__ bool segment_has(bool pred)
 // index of unexpanded (x, y).  The caller stores f.
 // The outlet's uy input (UY_SOURCE nodes, every step) and, on macro steps,
 // the four results of inner node (x, y) (unexpanded coordinates).
-template <class C>
-__device__ __forceinline__ void write_moments(const StepParams &p, uint32_t code, uint32_t x, uint32_t y, C rho, C jx,
+// R: the density's type (the 16-bit forms pass rho_ref + sigma in double).
+template <class C, class R = C>
+__device__ __forceinline__ void write_moments(const StepParams &p, uint32_t code, uint32_t x, uint32_t y, R rho, C jx,
                                               C jy, C usq)
 {
     if (code & LBM_CODE_UY_SOURCE)
@@ -486,16 +593,33 @@ __device__ __forceinline__ void write_moments(const StepParams &p, uint32_t code
     }
 }
 
+// r0: the 16-bit forms' rho_ref of the node's column; f then holds the
+// deviations g_d = f_d - w_d r0.  The equilibrium is linear in rho and
+// sum_d w_d c_d = 0, so the same outlet and collision on g, with rho_out - r0
+// for rho_out, give f' - w_d r0 (every operand a small deviation: FP32 then
+// resolves the velocity signal instead of rounding it against |f| ~ 0.1 - 0.44),
+// and the density is r0 + sum g.
 template <int L, int S>
 __device__ __forceinline__ void finish_node(const StepParams &p, uint32_t code, uint32_t x, uint32_t y,
-                                            typename Storage<S>::C f[9])
+                                            typename Storage<S>::C f[9], float r0 = 1.f)
 {
     using C = typename Storage<S>::C;
-    if (code & LBM_CODE_OUTLET)
-        zou_he_outlet<C, Storage<S>::kPromote>(f, C(p.rho_out), C(p.uy_in[y]));
-    C rho, jx, jy, usq;
-    collide<C, Storage<S>::kPromote, Storage<S>::kFull>(f, C(p.omega), rho, jx, jy, usq);
-    write_moments<C>(p, code, x, y, rho, jx, jy, usq);
+    if constexpr (shifted_storage<S>())
+    {
+        if (code & LBM_CODE_OUTLET)
+            zou_he_outlet<C>(f, C(p.rho_out) - r0, C(p.uy_in[y]));
+        C sigma, jx, jy, usq;
+        collide<C, false, false>(f, C(p.omega), sigma, jx, jy, usq);
+        write_moments<C, double>(p, code, x, y, double(r0) + double(sigma), jx, jy, usq);
+    }
+    else
+    {
+        if (code & LBM_CODE_OUTLET)
+            zou_he_outlet<C, Storage<S>::kPromote>(f, C(p.rho_out), C(p.uy_in[y]));
+        C rho, jx, jy, usq;
+        collide<C, Storage<S>::kPromote, Storage<S>::kFull>(f, C(p.omega), rho, jx, jy, usq);
+        write_moments<C>(p, code, x, y, rho, jx, jy, usq);
+    }
 }
 
 // ---- host side: the real-type parameters of a step -----------------------
@@ -507,6 +631,15 @@ inline double host_omega(int storage, double tau)
     if (storage == LBM_STORE_F32)
         return double(1.0f / float(tau));
     return 1.0 / tau;
+}
+
+// The initial rho_ref table of the 16-bit forms (H unexpanded columns,
+// ghosts included): rho_in at the inlet ghost, rho_out at the outlet ghost,
+// 1 (the rest state) in between.
+inline void host_reference(double rho_in, double rho_out, uint32_t H, float *out)
+{
+    for (uint32_t x = 0; x < H; ++x)
+        out[x] = x == 0u ? float(rho_in) : (x + 1u == H ? float(rho_out) : 1.f);
 }
 
 // maxwell_boltzmann_distribution (core/simulation.hpp:333-350) in the

@@ -74,6 +74,7 @@ struct lbm_ctx
     double *uycol = nullptr;  // 2 * V outlet uy history (ping-pong)
     double *snap = nullptr;   // 2 * N residual snapshot (lazy)
     double *pass = nullptr;   // 9 * P * Vx FP64 pass buffer of split steps (lazy)
+    double *stage = nullptr;  // population accessors' staging buffer (lazy, at most kStageDoubles)
     double *partials = nullptr;
     double *red_out = nullptr;
     uint32_t D;               // (H-2)*(V-2)
@@ -91,6 +92,14 @@ struct lbm_ctx
     bool snap_valid = false;
     double last_ms = 0.0;
     double feq_in[9], feq_out[9];
+    // 16-bit forms: rho_ref per unexpanded column (Hx + 64 floats, the tail
+    // 1), two tables used in turn; a step that changes the reference reads
+    // with ref[ref_cur] and writes with ref[1 - ref_cur]
+    float *ref[2] = {nullptr, nullptr};
+    int ref_cur = 0;
+    bool ref_switch = false;
+    double *ref_part = nullptr;  // column sums of the update (kRefChunks * Hx * 2)
+    uint32_t ref_len = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -121,6 +130,8 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
     if (x > p.Hx - 2u)
         return;
     const uint32_t y0 = 1u + blockIdx.y * R;
+    const Rt3 r3 = rt3_of<S>(p.ref_in, x);                  // the same column for every row
+    const float orel = rt_of<S>(p.ref_out, x) - r3.own();  // stores: against the written reference
     uint32_t pn[R], code[R];
     bool fluid[R], fill[R];
     C f[R][9];
@@ -139,14 +150,14 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
 #pragma unroll
     for (uint32_t r = 0; r < R; ++r)
         if (fluid[r])
-            pull<L, S>(static_cast<const T *>(p.src), pn[r], code[r], p.P, p.S, f[r]);
+            pull<L, S>(static_cast<const T *>(p.src), pn[r], code[r], p.P, p.S, f[r], r3);
 #pragma unroll
     for (uint32_t r = 0; r < R; ++r)
     {
         if (fluid[r])
-            finish_node<L, S>(p, code[r], x, y0 + r, f[r]);
+            finish_node<L, S>(p, code[r], x, y0 + r, f[r], r3.own());
         if (fluid[r] || fill[r])
-            store_node<L, S>(static_cast<T *>(p.dst), pn[r], p.S, f[r], p.sr);
+            store_node<L, S>(static_cast<T *>(p.dst), pn[r], p.S, f[r], p.sr, orel);
     }
 }
 
@@ -308,6 +319,9 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
     const uint32_t code = active ? p.code[pn] : 0u;
     const bool fluid = code & LBM_CODE_FLUID;
     const bool solid = code & LBM_CODE_SOLID;
+    // rho_ref of columns x - 1, x, x + 1 (16-bit forms), read and written
+    const Rt3 r3 = rt3_of<S>(p.ref_in, x0 + tx);
+    const float rto = rt_of<S>(p.ref_out, x0 + tx) - r3.own();  // stores: against the written reference
     C f[9] = {};
     if (fluid)
     {
@@ -315,12 +329,13 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
         for (int d = 0; d < 9; ++d)
         {
             const bool bb = (code >> d) & 1u;
+            const float rt = bb ? 0.f : r3.rel(cx_of(d));
             if constexpr (L == LBM_LAYOUT_COLLISION)
             {
                 const uint32_t r = bb ? ty + 1u : uint32_t(int32_t(ty) + 1 - cy_of(d));
                 const uint32_t i = bb ? tx + 1u : uint32_t(int32_t(tx) + 1 - cx_of(d));
                 const int dd = bb ? 8 - d : d;
-                f[d] = Storage<S>::load(stage + r * ROW + shift[r], 9u * i + uint32_t(dd), dd);
+                f[d] = Storage<S>::load(stage + r * ROW + shift[r], 9u * i + uint32_t(dd), dd, rt);
             }
             else
             {
@@ -328,15 +343,16 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
                 {
                     const uint32_t r = uint32_t(d / 3);
                     const uint32_t i = uint32_t(int32_t(tx) + 1 - cx_of(d));
-                    f[d] = Storage<S>::load(stage + r * ROW + shift[r], 3u * i + uint32_t(d % 3), d);
+                    f[d] = Storage<S>::load(stage + r * ROW + shift[r], 3u * i + uint32_t(d % 3), d, rt);
                 }
                 else if ((8 - d) / 3 == 1)
-                    f[d] = Storage<S>::load(stage + ROW + shift[1], 3u * (tx + 1u) + uint32_t((8 - d) % 3), 8 - d);
+                    f[d] = Storage<S>::load(stage + ROW + shift[1], 3u * (tx + 1u) + uint32_t((8 - d) % 3), 8 - d,
+                                            rt);
                 else
-                    f[d] = Storage<S>::load(src, at<L>(pn, uint32_t(8 - d), p.S), 8 - d);
+                    f[d] = Storage<S>::load(src, at<L>(pn, uint32_t(8 - d), p.S), 8 - d, rt);
             }
         }
-        finish_node<L, S>(p, code, x0 + tx, y, f);
+        finish_node<L, S>(p, code, x0 + tx, y, f, r3.own());
     }
     // which places of each node row are fluid (a wave is a node row: B = 64)
     __shared__ unsigned long long fluid_row[R];
@@ -347,7 +363,7 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
     if (__syncthreads_or(odd))
     {
         if (fluid || solid)
-            store_node<L, S>(dst, pn, p.S, f, p.sr);
+            store_node<L, S>(dst, pn, p.S, f, p.sr, rto);
         return;
     }
     // results through LDS (the staged ranges are free after the barrier above):
@@ -367,11 +383,11 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
                 if (p.sr)
                 {
                     hh = hh * 747796405u + 2891336453u;
-                    Storage<S>::store_sr(out, o, f[d], d, hh >> (S == LBM_STORE_F16_SHIFTED ? 19 : 16));
+                    Storage<S>::store_sr(out, o, f[d], d, rto, hh >> (S == LBM_STORE_F16_SHIFTED ? 19 : 16));
                     continue;
                 }
             }
-            Storage<S>::store(out, o, f[d], d);
+            Storage<S>::store(out, o, f[d], d, rto);
         }
     }
     __syncthreads();
@@ -422,14 +438,15 @@ __global__ __launch_bounds__(1024) void k_step_lptl(StepParams p)
     const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
     const bool fill = (code & LBM_CODE_SOLID) && segment;
     C f[9] = {};
+    const uint32_t y = pn / p.P, x = pn - y * p.P - p.X0;  // meaningful on fluid and solid places
     if (fluid)
     {
-        const uint32_t y = pn / p.P, x = pn - y * p.P - p.X0;
-        pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f);
-        finish_node<L, S>(p, code, x, y, f);
+        const Rt3 r3 = rt3_of<S>(p.ref_in, x);
+        pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f, r3);
+        finish_node<L, S>(p, code, x, y, f, r3.own());
     }
     if (fluid || fill)
-        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr);
+        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr, rt_of<S>(p.ref_out, x) - rt_of<S>(p.ref_in, x));
 }
 
 // npol: one lattice, one work-group per subdomain tile; every pull of the
@@ -449,14 +466,16 @@ __global__ __launch_bounds__(1024) void k_step_npol(StepParams p)
     const bool fluid = code & LBM_CODE_FLUID;
     const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
     const bool fill = (code & LBM_CODE_SOLID) && segment;
+    const uint32_t xu = x - x / (p.sub_h + 1u);
+    const Rt3 r3 = rt3_of<S>(p.ref_in, xu);
     C f[9] = {};
     if (fluid)
-        pull<L, S>(lat, pn, code, p.P, p.S, f);
+        pull<L, S>(lat, pn, code, p.P, p.S, f, r3);
     __syncthreads();
     if (fluid)
-        finish_node<L, S>(p, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
+        finish_node<L, S>(p, code, xu, y - y / (p.sub_v + 1u), f, r3.own());
     if (fluid || fill)
-        store_node<L, S>(lat, pn, p.S, f, p.sr);
+        store_node<L, S>(lat, pn, p.S, f, p.sr, rt_of<S>(p.ref_out, xu) - r3.own());
 }
 
 // nsol: one lattice, swap scheme (nsol_kernels.hpp:171-334).  Work-groups of
@@ -464,19 +483,21 @@ __global__ __launch_bounds__(1024) void k_step_npol(StepParams p)
 // populations 5..8 to the LDS slots of the neighbours that pull them and
 // reads populations 0..3 straight from the upstream nodes in HBM.  One
 // barrier separates all reads from the in-place writes (the reference uses
-// two LDS slots and three barriers; four slots need one).
+// two LDS slots and three barriers; four slots need one).  The LDS slots
+// carry the stored bits; the node that pulls a population decodes it, with
+// the rho_ref of column xu - c_x as every pull does.
 template <int L, int S>
 __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
     extern __shared__ unsigned char lds_raw[];
-    C *lds = reinterpret_cast<C *>(lds_raw);
+    T *lds = reinterpret_cast<T *>(lds_raw);
     T *lat = static_cast<T *>(p.dst);
 
     const uint32_t LP = p.sub_h + 4u;
     const uint32_t plane = (p.sub_v + 2u) * LP;
-    C *s5 = lds, *s6 = lds + plane, *s7 = lds + 2u * plane, *s8 = lds + 3u * plane;
+    T *s5 = lds, *s6 = lds + plane, *s7 = lds + 2u * plane, *s8 = lds + 3u * plane;
 
     const uint32_t lx = threadIdx.x + 1u, ly = threadIdx.y;
     const uint32_t x = blockIdx.x * (p.sub_h + 1u) + threadIdx.x;
@@ -488,32 +509,36 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
     const bool segment = segment_has<T>(fluid);  // every lane takes part in the ballot
     const bool fill = (code & LBM_CODE_SOLID) && segment;
 
+    const uint32_t xu = x - x / (p.sub_h + 1u);
+    const Rt3 r3 = rt3_of<S>(p.ref_in, xu);
     C f[9] = {};
     if (!(code & LBM_CODE_SOLID))
     {
-        const C o5 = Storage<S>::load(lat, at<L>(pn, 5, Sd), 5);
-        const C o6 = Storage<S>::load(lat, at<L>(pn, 6, Sd), 6);
-        const C o7 = Storage<S>::load(lat, at<L>(pn, 7, Sd), 7);
-        const C o8 = Storage<S>::load(lat, at<L>(pn, 8, Sd), 8);
-        s5[ly * LP + lx + 1u] = o5;
-        s6[(ly + 1u) * LP + lx - 1u] = o6;
-        s7[(ly + 1u) * LP + lx] = o7;
-        s8[(ly + 1u) * LP + lx + 1u] = o8;
+        const T b5 = lat[at<L>(pn, 5, Sd)], b6 = lat[at<L>(pn, 6, Sd)];
+        const T b7 = lat[at<L>(pn, 7, Sd)], b8 = lat[at<L>(pn, 8, Sd)];
+        s5[ly * LP + lx + 1u] = b5;
+        s6[(ly + 1u) * LP + lx - 1u] = b6;
+        s7[(ly + 1u) * LP + lx] = b7;
+        s8[(ly + 1u) * LP + lx + 1u] = b8;
         if (fluid)
         {
-            f[4] = Storage<S>::load(lat, at<L>(pn, 4, Sd), 4);
-            f[3] = (code & (1u << 3)) ? o5 : Storage<S>::load(lat, at<L>(pn + 1u, 3, Sd), 3);
-            f[2] = (code & (1u << 2)) ? o6 : Storage<S>::load(lat, at<L>(pn - 1u + P, 2, Sd), 2);
-            f[1] = (code & (1u << 1)) ? o7 : Storage<S>::load(lat, at<L>(pn + P, 1, Sd), 1);
-            f[0] = (code & (1u << 0)) ? o8 : Storage<S>::load(lat, at<L>(pn + 1u + P, 0, Sd), 0);
+            f[4] = Storage<S>::load(lat, at<L>(pn, 4, Sd), 4, 0.f);
+            f[3] = (code & (1u << 3)) ? Storage<S>::load(&b5, 0u, 5, 0.f)
+                                      : Storage<S>::load(lat, at<L>(pn + 1u, 3, Sd), 3, r3.rel(-1));
+            f[2] = (code & (1u << 2)) ? Storage<S>::load(&b6, 0u, 6, 0.f)
+                                      : Storage<S>::load(lat, at<L>(pn - 1u + P, 2, Sd), 2, r3.rel(1));
+            f[1] = (code & (1u << 1)) ? Storage<S>::load(&b7, 0u, 7, 0.f)
+                                      : Storage<S>::load(lat, at<L>(pn + P, 1, Sd), 1, 0.f);
+            f[0] = (code & (1u << 0)) ? Storage<S>::load(&b8, 0u, 8, 0.f)
+                                      : Storage<S>::load(lat, at<L>(pn + 1u + P, 0, Sd), 0, r3.rel(-1));
             if (code & (1u << 5))
-                f[5] = Storage<S>::load(lat, at<L>(pn, 3, Sd), 3);
+                f[5] = Storage<S>::load(lat, at<L>(pn, 3, Sd), 3, 0.f);
             if (code & (1u << 6))
-                f[6] = Storage<S>::load(lat, at<L>(pn, 2, Sd), 2);
+                f[6] = Storage<S>::load(lat, at<L>(pn, 2, Sd), 2, 0.f);
             if (code & (1u << 7))
-                f[7] = Storage<S>::load(lat, at<L>(pn, 1, Sd), 1);
+                f[7] = Storage<S>::load(lat, at<L>(pn, 1, Sd), 1, 0.f);
             if (code & (1u << 8))
-                f[8] = Storage<S>::load(lat, at<L>(pn, 0, Sd), 0);
+                f[8] = Storage<S>::load(lat, at<L>(pn, 0, Sd), 0, 0.f);
         }
     }
     __syncthreads();
@@ -521,17 +546,17 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
     {
         const uint32_t own = ly * LP + lx;
         if (!(code & (1u << 5)))
-            f[5] = s5[own];
+            f[5] = Storage<S>::load(s5, own, 5, r3.rel(1));
         if (!(code & (1u << 6)))
-            f[6] = s6[own];
+            f[6] = Storage<S>::load(s6, own, 6, r3.rel(-1));
         if (!(code & (1u << 7)))
-            f[7] = s7[own];
+            f[7] = Storage<S>::load(s7, own, 7, 0.f);
         if (!(code & (1u << 8)))
-            f[8] = s8[own];
-        finish_node<L, S>(p, code, x - x / (p.sub_h + 1u), y - y / (p.sub_v + 1u), f);
+            f[8] = Storage<S>::load(s8, own, 8, r3.rel(1));
+        finish_node<L, S>(p, code, xu, y - y / (p.sub_v + 1u), f, r3.own());
     }
     if (fluid || fill)
-        store_node<L, S>(lat, pn, p.S, f, p.sr);
+        store_node<L, S>(lat, pn, p.S, f, p.sr, rt_of<S>(p.ref_out, xu) - r3.own());
 }
 
 // npol and nsol with tiles at most one wave wide: one wave per tile, walking
@@ -616,6 +641,9 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
     const uint32_t xi = x - x / (sh + 1u);  // unexpanded column
     const uint32_t y00 = by * (sv + 1u) + 1u;
     const int lo = int(r * sh), hi = int(r * sh + sh);
+    // rho_ref of unexpanded columns xi - 1, xi, xi + 1 (16-bit forms), read and written
+    const Rt3 r3 = rt3_of<S>(p.ref_in, xi);
+    const float rto = rt_of<S>(p.ref_out, xi) - r3.own();  // stores: against the written reference
     const bool buf_left = i == 0u && bx > 0u;                // x - 1 is a buffer column
     const bool buf_right = i == sh - 1u && bx + 1u < p.cnt_h;  // x + 1 is one
     // lanes of a group's top row that the next group's bottom-row pulls of
@@ -677,7 +705,7 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
             }
             else
                 idx = at<L>(uint32_t(int32_t(pn) - cy * int32_t(P) - cx), uint32_t(d), Sd);
-            f[d] = Storage<S>::load(lat, idx, bb ? 8 - d : d);
+            f[d] = Storage<S>::load(lat, idx, bb ? 8 - d : d, bb ? 0.f : r3.rel(cx));
         }
     };
     // own old populations 6, 7, 8 of the group's top row (its last
@@ -689,9 +717,9 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
         if (r == R - 1u && y0 + R < sv && !(code & LBM_CODE_SOLID))
         {
             const uint32_t pn = place(x, y00 + y0 + r, P, p.X0);
-            c6 = Storage<S>::load(lat, at<L>(pn, 6u, Sd), 6);
-            c7 = Storage<S>::load(lat, at<L>(pn, 7u, Sd), 7);
-            c8 = Storage<S>::load(lat, at<L>(pn, 8u, Sd), 8);
+            c6 = Storage<S>::load(lat, at<L>(pn, 6u, Sd), 6, 0.f);
+            c7 = Storage<S>::load(lat, at<L>(pn, 7u, Sd), 7, 0.f);
+            c8 = Storage<S>::load(lat, at<L>(pn, 8u, Sd), 8, 0.f);
         }
     };
 
@@ -727,10 +755,10 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
                     {
                         const int sx = int(i) - cx_of(d);
                         if (!((code[h] >> d) & 1u) && sx >= 0 && sx < int(sh))
-                            f[h][d] = d == 6 ? k6 : (d == 7 ? k7 : k8);
+                            f[h][d] = rebase<S>(d == 6 ? k6 : (d == 7 ? k7 : k8), d, r3.rel(cx_of(d)));
                     }
                 }
-                finish_node<L, S>(p, code[h], xi, y - y / (sv + 1u), f[h]);
+                finish_node<L, S>(p, code[h], xi, y - y / (sv + 1u), f[h], r3.own());
             }
         }
         // keep_top's loads have returned before any lane overwrites what they read
@@ -743,7 +771,7 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
             const uint32_t row = y0 + h * R + r, y = y00 + row;
             const uint32_t pn = place(x, y, P, p.X0);
             T t[9];
-            node_bits<S>(f[h], pn, p.sr, t);
+            node_bits<S>(f[h], pn, p.sr, t, rto);
 #pragma unroll
             for (int d = 0; d < 9; ++d)
             {
@@ -918,10 +946,11 @@ __global__ __launch_bounds__(RED_BLOCK) void k_reduce(const typename Storage<S>:
         const uint32_t c = code[pn];
         if (!(c & LBM_CODE_FLUID))
             continue;
+        const float rt = rt_of<S>(g.ref, unexpanded_x(n % g.Hx, g));
         double f[9];
 #pragma unroll
         for (int d = 0; d < 9; ++d)
-            f[d] = Storage<S>::decode(lat[at<L>(pn, uint32_t(d), g.Sd)], d);
+            f[d] = Storage<S>::decode(lat[at<L>(pn, uint32_t(d), g.Sd)], d, rt);
         double rho = 0, jx = 0, jy = 0;
 #pragma unroll
         for (int d = 0; d < 9; ++d)
@@ -970,6 +999,59 @@ __global__ void k_reduce_final(const double *partials, int blocks, double *out)
     out[k] = s;
 }
 
+// ---------------------------------------------------------------------------
+// rho_ref of the 16-bit forms: the mean density of each column's fluid nodes,
+// every kRefInterval steps.  Row chunks give per-(chunk, column) sums, summed
+// per column in chunk order (deterministic); columns without a fluid node,
+// buffer columns and the ghost columns keep their value.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kRefInterval = 128;
+constexpr uint32_t kRefChunks = 64;
+
+template <int L, int S>
+__global__ void k_column_sums(const typename Storage<S>::T *lat, const uint16_t *code, Geom g, uint32_t Vx,
+                              double *part)
+{
+    const uint32_t xe = blockIdx.x * blockDim.x + threadIdx.x;
+    if (xe >= g.Hx)
+        return;
+    const uint32_t rows = (Vx + kRefChunks - 1u) / kRefChunks;
+    const uint32_t y0 = blockIdx.y * rows, y1 = min(Vx, y0 + rows);
+    const float rt = rt_of<S>(g.ref, unexpanded_x(xe, g));
+    double sum = 0.0, count = 0.0;
+    for (uint32_t y = y0; y < y1; ++y)
+    {
+        const uint32_t pn = place(xe, y, g.P, g.X0);
+        if (!(code[pn] & LBM_CODE_FLUID))
+            continue;
+        double rho = 0.0;
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            rho += Storage<S>::decode(lat[at<L>(pn, uint32_t(d), g.Sd)], d, rt);
+        sum += rho;
+        count += 1.0;
+    }
+    part[2ull * (size_t(blockIdx.y) * g.Hx + xe)] = sum;
+    part[2ull * (size_t(blockIdx.y) * g.Hx + xe) + 1] = count;
+}
+
+// ref_out: a copy of the current table, then the new column means
+[[maybe_unused]] static __global__ void k_column_ref(const double *part, Geom g, float *ref_out)
+{
+    const uint32_t xe = blockIdx.x * blockDim.x + threadIdx.x;
+    if (xe >= g.Hx)
+        return;
+    double sum = 0.0, count = 0.0;
+    for (uint32_t k = 0; k < kRefChunks; ++k)
+    {
+        sum += part[2ull * (size_t(k) * g.Hx + xe)];
+        count += part[2ull * (size_t(k) * g.Hx + xe) + 1];
+    }
+    const uint32_t xu = unexpanded_x(xe, g);
+    if (count > 0.0 && xu >= 1u && xu + 2u <= g.H && isfinite(sum))
+        ref_out[xu] = float(sum / count);
+}
+
 __global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
 {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -998,7 +1080,7 @@ __device__ __forceinline__ void split_xy(const StepParams &p, uint32_t pn, uint3
 }
 
 template <int L, int S>
-__global__ void k_split_stream(StepParams p, double *pass, uint32_t PV)
+__global__ void k_split_stream(StepParams p, double *pass, uint32_t PV, uint32_t buffered)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
@@ -1008,11 +1090,13 @@ __global__ void k_split_stream(StepParams p, double *pass, uint32_t PV)
     const uint32_t code = p.code[pn];
     if (!(code & LBM_CODE_FLUID))
         return;
+    uint32_t x, y;
+    split_xy(p, pn, buffered, x, y);
     C f[9];
-    pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f);
+    pull<L, S>(static_cast<const T *>(p.src), pn, code, p.P, p.S, f, rt3_of<S>(p.ref_in, x));
 #pragma unroll
     for (int d = 0; d < 9; ++d)
-        pass[size_t(d) * PV + pn] = double(f[d]);
+        pass[size_t(d) * PV + pn] = double(f[d]);  // 16-bit forms: deviations from the node's rho_ref
 }
 
 template <int S>
@@ -1031,7 +1115,10 @@ __global__ void k_split_outlet(StepParams p, double *pass, uint32_t PV, uint32_t
 #pragma unroll
     for (int d = 0; d < 9; ++d)
         f[d] = C(pass[size_t(d) * PV + pn]);
-    zou_he_outlet<C, Storage<S>::kPromote>(f, C(p.rho_out), C(p.uy_in[y]));
+    if constexpr (shifted_storage<S>())
+        zou_he_outlet<C>(f, C(p.rho_out) - rt_of<S>(p.ref_in, x), C(p.uy_in[y]));
+    else
+        zou_he_outlet<C, Storage<S>::kPromote>(f, C(p.rho_out), C(p.uy_in[y]));
     pass[pn] = double(f[0]);
     pass[3ull * PV + pn] = double(f[3]);
     pass[6ull * PV + pn] = double(f[6]);
@@ -1055,11 +1142,14 @@ __global__ void k_split_moments(StepParams p, double *pass, uint32_t PV, uint32_
         f[d] = C(pass[size_t(d) * PV + pn]);
     C rho, jx, jy, usq;
     moments<C, Storage<S>::kFull>(f, rho, jx, jy, usq);
-    write_moments<C>(p, code, x, y, rho, jx, jy, usq);
+    if constexpr (shifted_storage<S>())
+        write_moments<C, double>(p, code, x, y, double(rt_of<S>(p.ref_in, x)) + double(rho), jx, jy, usq);
+    else
+        write_moments<C>(p, code, x, y, rho, jx, jy, usq);
 }
 
 template <int L, int S>
-__global__ void k_split_collide(StepParams p, const double *pass, uint32_t PV)
+__global__ void k_split_collide(StepParams p, const double *pass, uint32_t PV, uint32_t buffered)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
@@ -1078,7 +1168,11 @@ __global__ void k_split_collide(StepParams p, const double *pass, uint32_t PV)
         collide<C, Storage<S>::kPromote, Storage<S>::kFull>(f, C(p.omega), rho, jx, jy, usq);
     }
     if (fluid || fill)
-        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr);
+    {
+        uint32_t x, y;
+        split_xy(p, pn, buffered, x, y);
+        store_node<L, S>(static_cast<T *>(p.dst), pn, p.S, f, p.sr, rt_of<S>(p.ref_out, x) - rt_of<S>(p.ref_in, x));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1112,6 +1206,8 @@ struct Impl
         p.uy_in = c->uycol + size_t(parity) * c->V;
         p.uy_out = c->uycol + size_t(1 - parity) * c->V;
         p.omega = host_omega(c->desc.storage, c->desc.relaxation_time);
+        p.ref_in = c->ref[c->ref_cur];
+        p.ref_out = c->ref_switch ? c->ref[1 - c->ref_cur] : p.ref_in;
         p.rho_out = c->desc.outlet_density;
         p.H = c->H;
         p.V = c->V;
@@ -1136,8 +1232,41 @@ struct Impl
         return p;
     }
 
+    // A step at an iteration that is a positive multiple of kRefInterval
+    // stores with the new column means of the lattice it reads (16-bit forms).
+    static int begin_reference(lbm_ctx *c)
+    {
+        if constexpr (shifted_storage<S>())
+        {
+            if (c->iteration == 0u || c->iteration % kRefInterval != 0u || c->ref_switch)
+                return LBM_OK;
+            float *next = c->ref[1 - c->ref_cur];
+            LBM_HIP_TRY(hipMemcpyAsync(next, c->ref[c->ref_cur], c->ref_len * sizeof(float), hipMemcpyDeviceToDevice,
+                                       c->stream));
+            const Geom g = geom(c);
+            hipLaunchKernelGGL((k_column_sums<L, S>), dim3((c->Hx + 255u) / 256u, kRefChunks), dim3(256), 0, c->stream,
+                               static_cast<const T *>(c->lat[c->cur]), c->code, g, c->Vx, c->ref_part);
+            hipLaunchKernelGGL(k_column_ref, dim3((c->Hx + 255u) / 256u), dim3(256), 0, c->stream, c->ref_part, g,
+                               next);
+            LBM_HIP_TRY(hipGetLastError());
+            c->ref_switch = true;
+        }
+        return LBM_OK;
+    }
+
+    static void end_reference(lbm_ctx *c)
+    {
+        if (c->ref_switch)
+        {
+            c->ref_cur ^= 1;
+            c->ref_switch = false;
+        }
+    }
+
     static int step(lbm_ctx *c, bool macros)
     {
+        if (int rc = begin_reference(c))
+            return rc;
         const StepParams p = params(c, macros);
         auto launch_staged = [&]() {
             if constexpr (nptl_staged<L, S>())
@@ -1224,6 +1353,7 @@ struct Impl
         if (c->two_lattice)
             c->cur ^= 1;
         c->iteration++;
+        end_reference(c);
         return LBM_OK;
     }
 
@@ -1232,6 +1362,9 @@ struct Impl
     static int split(lbm_ctx *c, int pass, double *buf)
     {
         c->side_valid = false;  // the passes change the lattice behind the side arrays
+        if (pass == LBM_PASS_STREAM)
+            if (int rc = begin_reference(c))
+                return rc;
         const StepParams p = params(c, true);
         const uint32_t PV = c->P * c->Vx;
         const dim3 grid((PV + 255u) / 256u), block(256);
@@ -1241,7 +1374,7 @@ struct Impl
         case LBM_PASS_STREAM:
             if (c->buffered)
                 refresh(c);
-            hipLaunchKernelGGL((k_split_stream<L, S>), grid, block, 0, c->stream, p, buf, PV);
+            hipLaunchKernelGGL((k_split_stream<L, S>), grid, block, 0, c->stream, p, buf, PV, buffered);
             break;
         case LBM_PASS_OUTLET:
             hipLaunchKernelGGL((k_split_outlet<S>), grid, block, 0, c->stream, p, buf, PV, buffered);
@@ -1250,11 +1383,12 @@ struct Impl
             hipLaunchKernelGGL((k_split_moments<S>), grid, block, 0, c->stream, p, buf, PV, buffered);
             break;
         default:
-            hipLaunchKernelGGL((k_split_collide<L, S>), grid, block, 0, c->stream, p, buf, PV);
+            hipLaunchKernelGGL((k_split_collide<L, S>), grid, block, 0, c->stream, p, buf, PV, buffered);
             LBM_HIP_TRY(hipGetLastError());
             if (c->two_lattice)
                 c->cur ^= 1;
             c->iteration++;
+            end_reference(c);
             return LBM_OK;
         }
         LBM_HIP_TRY(hipGetLastError());
@@ -1287,7 +1421,17 @@ struct Impl
         c->side_valid = true;
     }
 
-    static Geom geom(const lbm_ctx *c) { return Geom{c->Hx, c->N, c->P, c->X0, c->Sd}; }
+    static Geom geom(const lbm_ctx *c)
+    {
+        Geom g{c->Hx, c->N, c->P, c->X0, c->Sd};
+        g.ref = c->ref[c->ref_cur];
+        g.H = c->H;
+        g.buffered = c->buffered ? 1u : 0u;
+        g.sub_h = c->sub_h;
+        // the outlet ghost follows the last real column (BufferedNodeAccess of H - 2, plus one)
+        g.ghost_xe = c->buffered ? (c->H - 2u) + (c->H - 3u) / c->sub_h + 1u : c->H - 1u;
+        return g;
+    }
 
     static int init(lbm_ctx *c)
     {
@@ -1311,20 +1455,19 @@ struct Impl
         return LBM_OK;
     }
 
-    static int decode(lbm_ctx *c, double *dev_out)
+    // populations i0 .. i0 + count - 1 (flat layout order) to / from dev[0 .. count - 1]
+    static int decode(lbm_ctx *c, double *dev_out, uint32_t i0, uint32_t count)
     {
-        const uint32_t total = 9u * c->N;
-        hipLaunchKernelGGL((k_decode<L, S>), dim3((total + 255u) / 256u), dim3(256), 0, c->stream,
-                           static_cast<const T *>(c->lat[c->cur]), dev_out, geom(c));
+        hipLaunchKernelGGL((k_decode<L, S>), dim3((count + 255u) / 256u), dim3(256), 0, c->stream,
+                           static_cast<const T *>(c->lat[c->cur]), dev_out, geom(c), i0, count);
         LBM_HIP_TRY(hipGetLastError());
         return LBM_OK;
     }
 
-    static int encode(lbm_ctx *c, const double *dev_in)
+    static int encode(lbm_ctx *c, const double *dev_in, uint32_t i0, uint32_t count)
     {
-        const uint32_t total = 9u * c->N;
-        hipLaunchKernelGGL((k_encode<L, S>), dim3((total + 255u) / 256u), dim3(256), 0, c->stream,
-                           static_cast<T *>(c->lat[c->cur]), dev_in, geom(c));
+        hipLaunchKernelGGL((k_encode<L, S>), dim3((count + 255u) / 256u), dim3(256), 0, c->stream,
+                           static_cast<T *>(c->lat[c->cur]), dev_in, geom(c), i0, count);
         LBM_HIP_TRY(hipGetLastError());
         return LBM_OK;
     }
@@ -1352,8 +1495,8 @@ struct VTable
 {
     int (*step)(lbm_ctx *, bool);
     int (*init)(lbm_ctx *);
-    int (*decode)(lbm_ctx *, double *);
-    int (*encode)(lbm_ctx *, const double *);
+    int (*decode)(lbm_ctx *, double *, uint32_t, uint32_t);
+    int (*encode)(lbm_ctx *, const double *, uint32_t, uint32_t);
     int (*equilibrium)(lbm_ctx *, const double *, const double *, const double *);
     int (*reduce)(lbm_ctx *, int, double *);
     int (*split)(lbm_ctx *, int, double *);
@@ -1496,8 +1639,12 @@ void lbm_destroy(lbm_ctx *c)
     (void)hipFree(c->uycol);
     (void)hipFree(c->snap);
     (void)hipFree(c->pass);
+    (void)hipFree(c->stage);
     (void)hipFree(c->partials);
     (void)hipFree(c->red_out);
+    (void)hipFree(c->ref[0]);
+    (void)hipFree(c->ref[1]);
+    (void)hipFree(c->ref_part);
     if (c->ev0)
         (void)hipEventDestroy(c->ev0);
     if (c->ev1)
@@ -1619,6 +1766,15 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     CREATE_TRY(hipMalloc(&c->uycol, 2ull * c->V * sizeof(double)));
     CREATE_TRY(hipMalloc(&c->partials, size_t(RED_GRID) * RED_K * sizeof(double)));
     CREATE_TRY(hipMalloc(&c->red_out, RED_K * sizeof(double)));
+    if (desc->storage == LBM_STORE_F16_SHIFTED || desc->storage == LBM_STORE_BF16_SHIFTED)
+    {
+        // rho_ref tables (+64: the kernels read a column to either side of
+        // every lane's own, padding lanes included) and the update's sums
+        c->ref_len = std::max(c->Hx, c->H) + 64u;
+        CREATE_TRY(hipMalloc(&c->ref[0], c->ref_len * sizeof(float)));
+        CREATE_TRY(hipMalloc(&c->ref[1], c->ref_len * sizeof(float)));
+        CREATE_TRY(hipMalloc(&c->ref_part, 2ull * kRefChunks * c->Hx * sizeof(double)));
+    }
     // Data fills phase with -1 (simulation.cpp:424); Results fill density -1,
     // momentum 0 (simulation.cpp:407-410).
     CREATE_TRY(hipMemsetAsync(c->phase, 0xff, c->N, c->stream));
@@ -1660,6 +1816,21 @@ int lbm_get_phase(lbm_ctx *c, int8_t *phase, size_t n)
     return LBM_OK;
 }
 
+// The 16-bit forms' initial rho_ref table: the rest state's 1, the ghost
+// columns at rho_in and rho_out.
+static int reset_reference(lbm_ctx *c)
+{
+    c->ref_cur = 0;
+    c->ref_switch = false;
+    if (!c->ref[0])
+        return LBM_OK;
+    std::vector<float> ref(c->ref_len, 1.f);
+    host_reference(c->desc.inlet_density, c->desc.outlet_density, c->H, ref.data());
+    LBM_HIP_TRY(hipMemcpyAsync(c->ref[0], ref.data(), ref.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
 int lbm_init_populations(lbm_ctx *c)
 {
     if (!c)
@@ -1668,6 +1839,8 @@ int lbm_init_populations(lbm_ctx *c)
         return fail(LBM_ERR_STATE, "lbm_init_populations: lbm_set_phase must come first");
     c->cur = 0;
     c->side_valid = false;
+    if (int rc = reset_reference(c))
+        return rc;
     if (int rc = vtable(c).init(c))
         return rc;
     CodeParams cp{c->H,  c->V,  c->Hx, c->N, c->sub_h, c->sub_v, c->P, c->X0, c->buffered ? 1 : 0,
@@ -1842,12 +2015,25 @@ int lbm_get_pass_populations(lbm_ctx *c, double *values, size_t count)
     const size_t PV = size_t(c->P) * c->Vx;
     std::vector<double> phys(9 * PV);
     LBM_HIP_TRY(hipMemcpyAsync(phys.data(), c->pass, phys.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // the 16-bit forms' passes hold deviations from each node's rho_ref: add w_d rho_ref back
+    std::vector<float> ref(c->ref_len);
+    if (c->ref_len)
+        LBM_HIP_TRY(hipMemcpyAsync(ref.data(), c->ref[c->ref_cur], ref.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->stream));
     LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    static const float w[9] = {1.f / 36, 1.f / 9, 1.f / 36, 1.f / 9, 4.f / 9, 1.f / 9, 1.f / 36, 1.f / 9, 1.f / 36};
     for (uint32_t n = 0; n < c->N; ++n)
     {
-        const size_t pn = c->X0 + n % c->Hx + size_t(n / c->Hx) * c->P;
+        const uint32_t xe = n % c->Hx;
+        const size_t pn = c->X0 + xe + size_t(n / c->Hx) * c->P;
+        const uint32_t xu = c->buffered ? xe - xe / (c->sub_h + 1u) : xe;
         for (int d = 0; d < 9; ++d)
-            values[size_t(d) * c->N + n] = phys[size_t(d) * PV + pn];
+        {
+            double v = phys[size_t(d) * PV + pn];
+            if (c->ref_len && xu < c->ref_len)
+                v += double(w[d] * ref[xu]);
+            values[size_t(d) * c->N + n] = v;
+        }
     }
     return LBM_OK;
 }
@@ -1883,10 +2069,13 @@ int lbm_get_macros(lbm_ctx *c, double *rho, double *jx, double *jy, double *jabs
     return LBM_OK;
 }
 
+// iteration, lattice index, lattice count; the lattice(s); outlet history;
+// results; the 16-bit forms' rho_ref table
 static size_t state_bytes(const lbm_ctx *c)
 {
     const size_t lattices = c->two_lattice ? 2 : 1;
-    return 16 + lattices * c->lattice_elems * c->elem + 2ull * c->V * sizeof(double) + 4ull * c->D * sizeof(double);
+    return 16 + lattices * c->lattice_elems * c->elem + 2ull * c->V * sizeof(double) + 4ull * c->D * sizeof(double) +
+           size_t(c->ref_len) * sizeof(float);
 }
 
 int lbm_state_size(const lbm_ctx *c, size_t *bytes)
@@ -1918,6 +2107,10 @@ int lbm_get_state(lbm_ctx *c, void *buf, size_t bytes)
     LBM_HIP_TRY(hipMemcpyAsync(out + off, c->uycol, 2ull * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     off += 2ull * c->V * sizeof(double);
     LBM_HIP_TRY(hipMemcpyAsync(out + off, c->res, 4ull * c->D * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    off += 4ull * c->D * sizeof(double);
+    if (c->ref_len)
+        LBM_HIP_TRY(hipMemcpyAsync(out + off, c->ref[c->ref_cur], c->ref_len * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->stream));
     LBM_HIP_TRY(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
@@ -1945,6 +2138,11 @@ int lbm_set_state(lbm_ctx *c, const void *buf, size_t bytes)
     LBM_HIP_TRY(hipMemcpyAsync(c->uycol, in + off, 2ull * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
     off += 2ull * c->V * sizeof(double);
     LBM_HIP_TRY(hipMemcpyAsync(c->res, in + off, 4ull * c->D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    off += 4ull * c->D * sizeof(double);
+    c->ref_cur = 0;
+    c->ref_switch = false;
+    if (c->ref_len)
+        LBM_HIP_TRY(hipMemcpyAsync(c->ref[0], in + off, c->ref_len * sizeof(float), hipMemcpyHostToDevice, c->stream));
     LBM_HIP_TRY(hipStreamSynchronize(c->stream));
     c->iteration = iteration;
     c->frame_iteration = ~0ull;
@@ -1954,25 +2152,36 @@ int lbm_set_state(lbm_ctx *c, const void *buf, size_t bytes)
     return LBM_OK;
 }
 
+// The population accessors convert between the storage and FP64 through a
+// staging buffer of at most kStageDoubles that the context keeps after the
+// first call, one chunk at a time.
+constexpr uint32_t kStageDoubles = 1u << 22;  // 32 MiB
+
+static int ensure_stage(lbm_ctx *c)
+{
+    if (!c->stage)
+        LBM_HIP_TRY(hipMalloc(&c->stage, size_t(std::min(kStageDoubles, 9u * c->N)) * sizeof(double)));
+    return LBM_OK;
+}
+
 int lbm_get_populations(lbm_ctx *c, double *values, size_t count)
 {
     if (!c || !values)
         return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_populations: null argument");
     if (count != 9ull * c->N)
         return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_get_populations: count must be 9 * expanded node count");
-    double *tmp = nullptr;
-    LBM_HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
-    int rc = vtable(c).decode(c, tmp);
-    if (!rc)
+    if (int rc = ensure_stage(c))
+        return rc;
+    const VTable vt = vtable(c);
+    for (uint32_t i0 = 0; i0 < count; i0 += kStageDoubles)
     {
-        hipError_t e = hipMemcpyAsync(values, tmp, count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess)
-            rc = fail(LBM_ERR_DEVICE, std::string("lbm_get_populations: ") + hipGetErrorString(e));
+        const uint32_t n = uint32_t(std::min<size_t>(kStageDoubles, count - i0));
+        if (int rc = vt.decode(c, c->stage, i0, n))
+            return rc;
+        LBM_HIP_TRY(hipMemcpyAsync(values + i0, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        LBM_HIP_TRY(hipStreamSynchronize(c->stream));  // the next chunk reuses the staging buffer
     }
-    (void)hipFree(tmp);
-    return rc;
+    return LBM_OK;
 }
 
 int lbm_set_populations(lbm_ctx *c, const double *values, size_t count)
@@ -1983,24 +2192,23 @@ int lbm_set_populations(lbm_ctx *c, const double *values, size_t count)
         return fail(LBM_ERR_STATE, "lbm_set_populations: call lbm_init_populations first");
     if (count != 9ull * c->N)
         return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_populations: count must be 9 * expanded node count");
-    double *tmp = nullptr;
-    LBM_HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
+    if (int rc = ensure_stage(c))
+        return rc;
     c->side_valid = false;
-    hipError_t e = hipMemcpyAsync(tmp, values, count * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    int rc = e == hipSuccess ? vtable(c).encode(c, tmp)
-                             : fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
-    if (!rc && c->two_lattice)
+    const VTable vt = vtable(c);
+    for (uint32_t i0 = 0; i0 < count; i0 += kStageDoubles)
     {
-        e = hipMemcpyAsync(c->lat[1 - c->cur], c->lat[c->cur], c->lattice_elems * c->elem,
-                           hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess)
-            rc = fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
+        const uint32_t n = uint32_t(std::min<size_t>(kStageDoubles, count - i0));
+        LBM_HIP_TRY(hipMemcpyAsync(c->stage, values + i0, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if (int rc = vt.encode(c, c->stage, i0, n))
+            return rc;
+        LBM_HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    e = hipStreamSynchronize(c->stream);
-    if (!rc && e != hipSuccess)
-        rc = fail(LBM_ERR_DEVICE, std::string("lbm_set_populations: ") + hipGetErrorString(e));
-    (void)hipFree(tmp);
-    return rc;
+    if (c->two_lattice)
+        LBM_HIP_TRY(hipMemcpyAsync(c->lat[1 - c->cur], c->lat[c->cur], c->lattice_elems * c->elem,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
 }
 
 int lbm_set_equilibrium_field(lbm_ctx *c, const double *rho, const double *jx, const double *jy, size_t count)

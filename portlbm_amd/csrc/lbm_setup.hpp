@@ -12,11 +12,40 @@ namespace lbm_dev {
 struct Geom
 {
     uint32_t Hx, N, P, X0, Sd;
+    const float *ref;   // rho_ref per unexpanded column (16-bit forms)
+    uint32_t H;         // unexpanded width, ghosts included
+    uint32_t buffered;  // npol / nsol: buffer columns every sub_h + 1
+    uint32_t sub_h;
+    uint32_t ghost_xe;  // expanded column of the outlet ghost
 };
 
 __device__ __forceinline__ uint32_t place_logical(uint32_t n, const Geom &g)
 {
     return place(n % g.Hx, n / g.Hx, g.P, g.X0);
+}
+
+// Unexpanded column of expanded column xe: a buffer column maps to the real
+// column on its left, the outlet ghost to H - 1 (it follows the last real
+// column directly, also where that puts it on a buffer column's place).
+__device__ __forceinline__ uint32_t unexpanded_x(uint32_t xe, const Geom &g)
+{
+    if (!g.buffered)
+        return xe;
+    if (xe >= g.ghost_xe)
+        return g.H - 1u + (xe - g.ghost_xe);
+    return xe - xe / (g.sub_h + 1u);
+}
+
+// rho_ref of the node whose population slot (xe, d) holds: a buffer column
+// carries, in directions 0, 3, 6, the populations of the real node to its
+// right (general/buffered.hpp:133-152), in the others those of the node to its left.
+template <int S>
+__device__ __forceinline__ float slot_rt(uint32_t xe, int d, const Geom &g)
+{
+    uint32_t xu = unexpanded_x(xe, g);
+    if (g.buffered && xe != 0u && xe < g.ghost_xe && xe % (g.sub_h + 1u) == 0u && cx_of(d) == -1)
+        xu += 1u;
+    return rt_of<S>(g.ref, xu);
 }
 
 template <int L, int S>
@@ -28,7 +57,7 @@ __global__ void k_init_rest(typename Storage<S>::T *lat, Geom g)
     const uint32_t pn = place_logical(n, g);
 #pragma unroll
     for (int d = 0; d < 9; ++d)
-        lat[at<L>(pn, uint32_t(d), g.Sd)] = Storage<S>::encode(weight<double>(d), d);
+        lat[at<L>(pn, uint32_t(d), g.Sd)] = Storage<S>::encode(weight<double>(d), d, slot_rt<S>(n % g.Hx, d, g));
 }
 
 struct InoutValues
@@ -51,9 +80,10 @@ __global__ void k_init_inout(typename Storage<S>::T *lat, uint32_t H, uint32_t V
     const uint32_t xe = buffered ? x + (x - 1u) / sub_h : x;
     const uint32_t ye = buffered ? y + (y - 1u) / sub_v : y;
     const uint32_t node = place(side ? xe + 1u : xe - 1u, ye, g.P, g.X0);
+    const float rt = rt_of<S>(g.ref, side ? H - 1u : 0u);
 #pragma unroll
     for (int d = 0; d < 9; ++d)
-        lat[at<L>(node, uint32_t(d), g.Sd)] = Storage<S>::encode(vals.v[side * 9 + d], d);
+        lat[at<L>(node, uint32_t(d), g.Sd)] = Storage<S>::encode(vals.v[side * 9 + d], d, rt);
 }
 
 struct CodeParams
@@ -65,7 +95,7 @@ struct CodeParams
 
 __device__ __forceinline__ uint32_t map_axis(uint32_t u, uint32_t last, uint32_t sub, int buffered)
 {
-    // unexpanded coordinate -> expanded (BufferedNodeAccess, access.hpp:353-362);
+    // unexpanded coordinate -> expanded (BufferedNodeAccess, access.hpp:292-301);
     // the high ghost sits right after the last real node
     if (!buffered)
         return u;
@@ -149,26 +179,28 @@ __device__ __forceinline__ void split_flat(uint32_t i, uint32_t N, uint32_t &n, 
     }
 }
 
+// Populations i0 .. i0 + count - 1 of the reference's flat layout order, to
+// or from out[0 .. count - 1].
 template <int L, int S>
-__global__ void k_decode(const typename Storage<S>::T *lat, double *out, Geom g)
+__global__ void k_decode(const typename Storage<S>::T *lat, double *out, Geom g, uint32_t i0, uint32_t count)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 9u * g.N)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count)
         return;
     uint32_t n, d;
-    split_flat<L>(i, g.N, n, d);
-    out[i] = Storage<S>::decode(lat[at<L>(place_logical(n, g), d, g.Sd)], int(d));
+    split_flat<L>(i0 + k, g.N, n, d);
+    out[k] = Storage<S>::decode(lat[at<L>(place_logical(n, g), d, g.Sd)], int(d), slot_rt<S>(n % g.Hx, int(d), g));
 }
 
 template <int L, int S>
-__global__ void k_encode(typename Storage<S>::T *lat, const double *in, Geom g)
+__global__ void k_encode(typename Storage<S>::T *lat, const double *in, Geom g, uint32_t i0, uint32_t count)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 9u * g.N)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count)
         return;
     uint32_t n, d;
-    split_flat<L>(i, g.N, n, d);
-    lat[at<L>(place_logical(n, g), d, g.Sd)] = Storage<S>::encode(in[i], int(d));
+    split_flat<L>(i0 + k, g.N, n, d);
+    lat[at<L>(place_logical(n, g), d, g.Sd)] = Storage<S>::encode(in[k], int(d), slot_rt<S>(n % g.Hx, int(d), g));
 }
 
 // Equilibrium of a given (rho, j) field on every fluid node (warm start).
@@ -190,12 +222,13 @@ __global__ void k_set_equilibrium(typename Storage<S>::T *lat, const uint16_t *c
     const uint32_t r = (x - 1u) + (y - 1u) * (H - 2u);
     const C rh = C(rho[r]), ux = C(jx[r]), uy = C(jy[r]);
     const C usq = ux * ux + uy * uy;
+    const float rt = rt_of<S>(g.ref, x);
 #pragma unroll
     for (int d = 0; d < 9; ++d)
     {
         const C cu = C(cx_of(d)) * ux + C(cy_of(d)) * uy;
         const C feq = weight<C>(d) * (rh + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
-        Storage<S>::store(lat, at<L>(pn, uint32_t(d), g.Sd), feq, d);
+        Storage<S>::store(lat, at<L>(pn, uint32_t(d), g.Sd), feq, d, rt);
     }
 }
 

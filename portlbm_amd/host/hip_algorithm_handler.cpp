@@ -355,6 +355,8 @@ void HipAlgorithmHandler::block_until_finished()
 namespace
 {
 constexpr char kSnapshotMagic[8] = {'P', 'L', 'B', 'M', 'S', 'N', 'A', 'P'};
+// 2: the 16-bit forms store f_d - w_d rho_lin(x) (1: f_d - w_d)
+constexpr std::uint32_t kSnapshotVersion = 2;
 
 // settings that must agree between a snapshot and the handler loading it
 file_interaction::JsonValue comparable_settings(const core::Properties &p)
@@ -383,7 +385,7 @@ void HipAlgorithmHandler::save_snapshot(const std::string &path) const
     std::ofstream out(path, std::ios::binary);
     if (!out)
         throw exceptions::Exception("cannot write snapshot \"" + path + "\"");
-    const std::uint32_t version = 1, len = static_cast<std::uint32_t>(text.size());
+    const std::uint32_t version = kSnapshotVersion, len = static_cast<std::uint32_t>(text.size());
     out.write(kSnapshotMagic, 8);
     out.write(reinterpret_cast<const char *>(&version), 4);
     out.write(reinterpret_cast<const char *>(&len), 4);
@@ -405,7 +407,7 @@ void HipAlgorithmHandler::load_snapshot(const std::string &path)
     in.read(magic, 8);
     in.read(reinterpret_cast<char *>(&version), 4);
     in.read(reinterpret_cast<char *>(&len), 4);
-    if (!in || std::memcmp(magic, kSnapshotMagic, 8) != 0 || version != 1 || len > (1u << 24))
+    if (!in || std::memcmp(magic, kSnapshotMagic, 8) != 0 || version != kSnapshotVersion || len > (1u << 24))
         throw exceptions::Exception("\"" + path + "\" is not a snapshot of this library");
     std::string text(len, '\0');
     in.read(text.data(), len);

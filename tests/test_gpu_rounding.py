@@ -36,7 +36,7 @@ def test_stochastic_rounding_is_deterministic_and_differs_from_nearest():
     assert not np.array_equal(a[1], c[1])
 
 
-@pytest.mark.parametrize("storage,bound", [("f16-shifted", 3e-3), ("bf16-shifted", 3e-2)])
+@pytest.mark.parametrize("storage,bound", [("f16-shifted", 1.5e-3), ("bf16-shifted", 1e-2)])
 def test_stochastic_rounding_beats_nearest_on_a_steady_flow(storage, bound):
     ref = _run("f64", "nearest")
     sr = _u_err(_run(storage, "stochastic"), ref)
