@@ -106,9 +106,12 @@ def cpu_baseline(algorithm: str, layout: str, budget_s: float = 15.0) -> dict:
                       f"{steps} steps after 1 warm-up step, {dt:.1f} s"}
 
 
-def measured_traffic(algorithm: str, layout: str, storage: str) -> float | None:
+def measured_traffic(algorithm: str, layout: str, storage: str, scenario: str = "porous-circles") -> float | None:
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if it
-    covers this kernel and workload."""
+    covers this kernel and workload (the summaries are of the porous
+    channel)."""
+    if scenario != "porous-circles":
+        return None
     try:
         data = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
@@ -389,7 +392,7 @@ def main() -> int:
         "mlups_all_nodes": round(INNER_H * INNER_V * args.steps / secs / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": measured_traffic(args.algorithm, args.layout, args.storage)},
+                     "traffic": measured_traffic(args.algorithm, args.layout, args.storage, args.scenario)},
         "kernel_ms": round(launch_ms, 5),
         "bytes_per_fluid_node": bytes_per_node,
         "mass": r["mass"],
