@@ -120,6 +120,25 @@ def measured_traffic(algorithm: str, layout: str, storage: str, scenario: str = 
     return None if rec is None else float(rec["bytes_per_launch"])
 
 
+def kernel_of(algorithm: str, layout: str, storage: str) -> str:
+    """The step kernel the library launches for a cell of the default
+    workload (work-group 1024 for npol / nsol: 32 x 32 and 30 x 31 tiles;
+    256 for lptl's 1-D kernel).
+    lptl runs nptl's kernels where those are faster: the staged kernel in the
+    collision layout and the FP64/FP32 bundle layout, the row kernel in
+    16-bit storage; its own 1-D kernel only in the FP64/FP32 stream layout."""
+    sixteen = storage in ("f16-shifted", "bf16-shifted")
+    if algorithm in ("nptl", "lptl"):
+        if layout == "collision" or (layout == "bundle" and not sixteen):
+            return "k_step_nptl_staged"
+        if algorithm == "lptl" and not sixteen:
+            return "k_step_lptl"
+        return "k_step_nptl"
+    if algorithm == "npol" and layout == "collision" and storage == "f64":
+        return "k_step_npol"
+    return "k_step_tile_sweep"
+
+
 def sweep(out_path: str, steps: int, warmup: int, cpu_budget_s: float = 0.0) -> None:
     """Every algorithm x layout x storage on the default workload.  With
     cpu_budget_s > 0 the CPU oracle (FP64, the reference's pass structure) is
@@ -138,7 +157,8 @@ def sweep(out_path: str, steps: int, warmup: int, cpu_budget_s: float = 0.0) -> 
                 r = time_gpu(workload(algorithm, layout, storage, steps), steps, warmup)
                 mflups = r["fluid"] * steps / (r["dev_ms"] * 1e-3) / 1e6
                 gbps = mflups * 1e6 * 2 * 9 * storage_bytes(storage) / 1e9
-                row = {"algorithm": algorithm, "layout": layout, "storage": storage, "mflups": round(mflups, 1),
+                row = {"algorithm": algorithm, "layout": layout, "storage": storage,
+                       "kernel": kernel_of(algorithm, layout, storage), "mflups": round(mflups, 1),
                        "mlups_all": round(INNER_H * INNER_V * steps / (r["dev_ms"] * 1e-3) / 1e6, 1),
                        "gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4),
                        "ms_per_step": round(r["dev_ms"] / steps, 4)}
