@@ -309,31 +309,38 @@ def steady_precision_study(out_path: str, inner_h: int = 1024, inner_v: int = 25
     Path(out_path).write_text(json.dumps({"workload": workload, "rows": rows}, indent=1))
 
 
-def batched(out_path: str, steps: int, count: int = 512) -> dict:
+def batched(out_path: str, steps: int, count: int = 512,
+            modes=("per-domain", "persistent", "graph")) -> dict:
     """512 uneven porous channels in FP32 (portlbm_amd.workloads.batch_channels):
     per-domain launches vs one persistent launch per step vs the whole run in
-    one hipGraph with autotuned tiles."""
+    one hipGraph with autotuned tiles (`modes` picks which are timed)."""
     docs = workloads.batch_channels(count, steps=steps)
     rows = {}
     with pa.Batch(docs, storage="f32") as b:
         fluid = b.fluid_nodes()
         nodes = sum((d["domain"]["horizontalNodes"]) * (d["domain"]["verticalNodes"]) for d in docs)
         for mode in ("per-domain", "persistent"):
+            if mode not in modes:
+                continue
             b.init()
             ms = b.run(steps, mode)
             rows[mode] = {"ms": round(ms, 2), "mflups": round(fluid * steps / (ms * 1e-3) / 1e6, 1)}
             print(json.dumps({mode: rows[mode]}), file=sys.stderr, flush=True)
-        tune_ms = b.autotune(steps_per_trial=20)
-        b.run(steps, "graph")  # capture + first replay
-        b.init()
-        ms = b.run(steps, "graph")
-        rows["graph+autotune"] = {"ms": round(ms, 2), "mflups": round(fluid * steps / (ms * 1e-3) / 1e6, 1),
-                                  "tile_widths_by_class": b.tiles(), "autotune_best_ms_per_step": round(tune_ms, 4)}
-        print(json.dumps({"graph+autotune": rows["graph+autotune"]}), file=sys.stderr, flush=True)
+        if "graph" in modes:
+            tune_ms = b.autotune(steps_per_trial=20)
+            b.run(steps, "graph")  # capture + first replay
+            b.init()
+            ms = b.run(steps, "graph")
+            rows["graph+autotune"] = {"ms": round(ms, 2), "mflups": round(fluid * steps / (ms * 1e-3) / 1e6, 1),
+                                      "tile_widths_by_class": b.tiles(),
+                                      "autotune_best_ms_per_step": round(tune_ms, 4)}
+            print(json.dumps({"graph+autotune": rows["graph+autotune"]}), file=sys.stderr, flush=True)
     result = {"workload": f"{count} porous channels nx U[64,1024] ny U[32,256], FP32, {steps} steps",
-              "domains": count, "inner_nodes": nodes, "fluid_nodes": fluid, "modes": rows,
-              "speedup_graph_vs_per_domain": round(rows["per-domain"]["ms"] / rows["graph+autotune"]["ms"], 2),
-              "gbps_graph": round(rows["graph+autotune"]["mflups"] * 1e6 * 72 / 1e9, 1)}
+              "domains": count, "inner_nodes": nodes, "fluid_nodes": fluid, "modes": rows}
+    if "per-domain" in rows and "graph+autotune" in rows:
+        result["speedup_graph_vs_per_domain"] = round(rows["per-domain"]["ms"] / rows["graph+autotune"]["ms"], 2)
+    if "graph+autotune" in rows:
+        result["gbps_graph"] = round(rows["graph+autotune"]["mflups"] * 1e6 * 72 / 1e9, 1)
     Path(out_path).write_text(json.dumps(result, indent=1))
     return result
 
@@ -362,6 +369,8 @@ def main() -> int:
                     help="reduced-precision storage against FP64 at the steady state of a 1024x256 porous channel")
     ap.add_argument("--batched", metavar="OUT.json")
     ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--batch-modes", default="per-domain,persistent,graph",
+                    help="comma list of per-domain, persistent, graph")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -380,7 +389,7 @@ def main() -> int:
     if args.steady_study:
         steady_precision_study(args.steady_study)
     if args.batched:
-        batched(args.batched, args.batch_steps)
+        batched(args.batched, args.batch_steps, modes=tuple(args.batch_modes.split(",")))
 
     doc = workload(args.algorithm, args.layout, args.storage, args.steps, args.scenario)
     r = time_gpu(doc, args.steps, args.warmup)
