@@ -356,7 +356,7 @@ namespace
 {
 constexpr char kSnapshotMagic[8] = {'P', 'L', 'B', 'M', 'S', 'N', 'A', 'P'};
 // 2: the 16-bit forms store f_d - w_d rho_lin(x) (1: f_d - w_d)
-constexpr std::uint32_t kSnapshotVersion = 2;
+constexpr std::uint32_t kSnapshotVersion = 3;
 
 // settings that must agree between a snapshot and the handler loading it
 file_interaction::JsonValue comparable_settings(const core::Properties &p)
