@@ -4,12 +4,13 @@
 // loss sums.  The network has no counterpart in the reference.
 //
 // Kernels (NCHW float32, 256-thread blocks):
-//   k_conv3x3      16x16 output pixels x 16 output channels per block; the
-//                  input tile with its 1-node halo staged in LDS in chunks of
-//                  8 channels; weights pre-packed [Cin][9][Cout16] so that the
-//                  16 weights of one (channel, tap) are wave-uniform scalar
-//                  loads feeding 16 FMAs per LDS read.  The data gradient is
-//                  the same kernel on flipped, transposed weights.
+//   k_conv3x3_f32  implicit GEMM on the matrix cores in FP32
+//                  (v_mfma_f32_16x16x4f32): 4 output rows x 16 pixels per
+//                  block (one row per wave) for all output channels; the input
+//                  tile with its 1-node halo staged in LDS; weights packed
+//                  [Cout16][K16], K = 9 Cin in the weights' own (c, tap) order.
+//                  The data gradient is the same kernel on flipped, transposed
+//                  weights.
 //   k_conv_wgrad   16 x 16 (out, in) channel pairs per block, each thread one
 //                  pair's 9 taps + bias over a sliding 3x3 register window;
 //                  per-slice partials summed in two fixed-order stages
@@ -37,8 +38,6 @@ constexpr int kCout[kConvs] = {16, 16, 32, 32, 48, 48, 64, 64, 64, 48, 32, 16, 1
 constexpr int kTile = 16;                    // output tile edge (pixels)
 constexpr int kHalo = kTile + 2;             // input tile edge with halo
 constexpr int kHaloStride = kHalo * kHalo + 1;  // LDS floats per channel (odd: spreads banks)
-constexpr int kCiChunk = 8;                  // input channels per LDS stage (forward)
-constexpr int kCoT = 16;                     // output channels per block
 constexpr int kWgT = 16;                     // channel tile of the weight gradient
 constexpr int kGStride = kTile * kTile + 1;  // LDS floats per gradient channel
 
@@ -49,62 +48,96 @@ inline int round16(int c) { return (c + 15) / 16 * 16; }
 // Resolution level (0 = full, 4 = 1/16) at which conv k runs.
 inline int conv_level(int k) { return k < 8 ? k / 2 : k == 8 ? 4 : k == 13 ? 0 : 12 - k; }
 
-// y[b, o, i, j] = act(bias[o] + sum_{c,k} wp[(c*9 + k)*CoutP + o] * x[b, c, i+ky-1, j+kx-1])
-__global__ __launch_bounds__(256) void k_conv3x3(const float *__restrict__ x, const float *__restrict__ wp,
-                                                 const float *__restrict__ bias, float *__restrict__ y, int Cin,
-                                                 int Cout, int CoutP, int H, int W, int tiles_x, int relu)
+// ---- FP32 convolution on the matrix cores ---------------------------------
+// y[b][o][p] = act(bias[o] + sum_k Wm[o][k] X[k][p]), k = c*9 + tap, with
+// v_mfma_f32_16x16x4f32 (A 16x4, B 4x16, FP32 products and accumulation).
+// Lane layout (the same as the BF16 instruction's, one element per lane):
+// A[i][kk] in lane i + 16 kk, B[kk][j] in lane j + 16 kk, D[i][j] in lane
+// j + 16 (i/4), element i%4.  Which k an operand slot kk carries is free as
+// long as A and B agree: in a 16-wide chunk at kc, MFMA s of the chunk gives
+// slot kk the k = kc + 4 kk + s, so that each lane reads its four A values
+// as one 16-byte load.  A 256-thread block computes 16 pixels of 4 rows (one
+// row per wave) for all output channels (at most 64: four 16-row tiles).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaRows = 4;  // output rows per block (one per wave)
+constexpr int kMfmaCols = 16;
+constexpr int kMaxCoTiles = 4;  // Cout <= 64
+
+__global__ __launch_bounds__(256) void k_conv3x3_f32(const float *__restrict__ x, const float *__restrict__ wm,
+                                                     const float *__restrict__ bias, float *__restrict__ y, int Cin,
+                                                     int Cout, int Kp, int H, int W, int tiles_x, int relu)
 {
-    __shared__ float tile[kCiChunk * kHaloStride];
-    const int b = blockIdx.z;
-    const int co0 = blockIdx.y * kCoT;
-    const int tx0 = int(blockIdx.x % unsigned(tiles_x)) * kTile, ty0 = int(blockIdx.x / unsigned(tiles_x)) * kTile;
-    const int lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
-    float acc[kCoT];
-#pragma unroll
-    for (int j = 0; j < kCoT; ++j)
-        acc[j] = 0.f;
+    constexpr int TR = kMfmaRows + 2, TC = kMfmaCols + 2;
+    extern __shared__ float tile_f[];  // Cin x TR x TC
+    const int b = blockIdx.y;
+    const int x0 = int(blockIdx.x % unsigned(tiles_x)) * kMfmaCols;
+    const int y0 = int(blockIdx.x / unsigned(tiles_x)) * kMfmaRows;
     const float *xb = x + size_t(b) * Cin * H * W;
-    for (int c0 = 0; c0 < Cin; c0 += kCiChunk)
+    for (int i = threadIdx.x; i < Cin * TR * TC; i += 256)
     {
-        const int nc = min(kCiChunk, Cin - c0);
-        __syncthreads();
-        for (int i = threadIdx.x; i < nc * kHalo * kHalo; i += 256)
+        const int c = i / (TR * TC), r = (i / TC) % TR, col = i % TC;
+        const int yy = y0 - 1 + r, xx = x0 - 1 + col;
+        float v = 0.f;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+            v = xb[(size_t(c) * H + yy) * W + xx];
+        tile_f[i] = v;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int j = lane % 16, kq = 4 * (lane / 16);
+    const int co_tiles = (Cout + 15) / 16;
+    const int K = 9 * Cin;
+    f32x4 acc[kMaxCoTiles];
+#pragma unroll
+    for (int t = 0; t < kMaxCoTiles; ++t)
+        acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < Kp; kc += 16)
+    {
+        float bv[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
         {
-            const int c = i / (kHalo * kHalo), r = i % (kHalo * kHalo);
-            const int yy = ty0 + r / kHalo - 1, xx = tx0 + r % kHalo - 1;
+            const int k = kc + kq + s;
             float v = 0.f;
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                v = xb[(size_t(c0 + c) * H + yy) * W + xx];
-            tile[c * kHaloStride + r] = v;
-        }
-        __syncthreads();
-        for (int c = 0; c < nc; ++c)
-        {
-            const float *wc = wp + size_t(c0 + c) * 9 * CoutP + co0;
-            const float *tc = tile + c * kHaloStride + ly * kHalo + lx;
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
+            if (k < K)
             {
-                const float v = tc[(k / 3) * kHalo + k % 3];
+                const int c = k / 9, tap = k - 9 * c;
+                v = tile_f[(c * TR + wave + tap / 3) * TC + j + tap % 3];
+            }
+            bv[s] = v;
+        }
 #pragma unroll
-                for (int j = 0; j < kCoT; ++j)
-                    acc[j] = fmaf(wc[k * CoutP + j], v, acc[j]);
+        for (int t = 0; t < kMaxCoTiles; ++t)
+        {
+            if (t < co_tiles)
+            {
+                const f32x4 av = *reinterpret_cast<const f32x4 *>(wm + size_t(t * 16 + j) * Kp + kc + kq);
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], bv[s], acc[t], 0, 0, 0);
             }
         }
     }
-    const int oy = ty0 + ly, ox = tx0 + lx;
+    const int oy = y0 + wave, ox = x0 + j;
     if (oy >= H || ox >= W)
         return;
 #pragma unroll
-    for (int j = 0; j < kCoT; ++j)
+    for (int t = 0; t < kMaxCoTiles; ++t)
     {
-        const int co = co0 + j;
-        if (co < Cout)
+        if (t >= co_tiles)
+            break;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
         {
-            float s = acc[j] + (bias ? bias[co] : 0.f);
-            if (relu)
-                s = s > 0.f ? s : 0.f;
-            y[((size_t(b) * Cout + co) * H + oy) * W + ox] = s;
+            const int co = t * 16 + kq + v;
+            if (co < Cout)
+            {
+                float r = acc[t][v] + (bias ? bias[co] : 0.f);
+                if (relu)
+                    r = r > 0.f ? r : 0.f;
+                y[((size_t(b) * Cout + co) * H + oy) * W + ox] = r;
+            }
         }
     }
 }
@@ -263,11 +296,6 @@ __global__ void k_wgrad_final(const double *__restrict__ part2, float *__restric
 // im2col fragment of a 16-wide K chunk once and reuses it for every 16-channel
 // output tile; weights come packed as BF16 [Cout16][K16] (zero padded).
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kMfmaRows = 4;  // output rows per block (one per wave)
-constexpr int kMfmaCols = 16;
-constexpr int kMaxCoTiles = 4;  // Cout <= 64
 
 __device__ __forceinline__ short f32_to_bf16_rne(float f)
 {
@@ -365,16 +393,19 @@ __global__ void k_pack_bf16(const float *__restrict__ w, short *__restrict__ wb,
     wb[size_t(o) * Kp + k] = f32_to_bf16_rne(w[idx]);
 }
 
-// wp[(c*9 + k)*CoutP + o] = w[o][c][k];  wt[(o*9 + 8-k)*CinP + c] = w[o][c][k]
-__global__ void k_pack(const float *__restrict__ w, float *__restrict__ wp, float *__restrict__ wt, int Cin, int Cout,
-                       int CinP, int CoutP)
+// wm[o*Kp + c*9 + k] = w[o][c][k] (Kp = round16(9 Cin));
+// wt[c*KpT + o*9 + 8-k] = w[o][c][k] (KpT = round16(9 Cout)): the data
+// gradient's weights, input and output channels swapped, taps flipped.
+// Padding rows and columns stay zero.
+__global__ void k_pack(const float *__restrict__ w, float *__restrict__ wm, float *__restrict__ wt, int Cin, int Cout,
+                       int Kp, int KpT)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= Cout * Cin * 9)
         return;
     const int o = idx / (Cin * 9), c = (idx / 9) % Cin, k = idx % 9;
-    wp[(size_t(c) * 9 + k) * CoutP + o] = w[idx];
-    wt[(size_t(o) * 9 + (8 - k)) * CinP + c] = w[idx];
+    wm[size_t(o) * Kp + c * 9 + k] = w[idx];
+    wt[size_t(c) * KpT + o * 9 + (8 - k)] = w[idx];
 }
 
 __global__ void k_pool2(const float *__restrict__ x, float *__restrict__ y, int BC, int H, int W)
@@ -595,13 +626,23 @@ int pack_all(lbm_surrogate *s)
     {
         const int n = kCout[k] * kCin[k] * 9;
         hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->params + s->w_off[k], s->wp[k],
-                           s->wt[k], kCin[k], kCout[k], round16(kCin[k]), round16(kCout[k]));
+                           s->wt[k], kCin[k], kCout[k], round16(9 * kCin[k]), round16(9 * kCout[k]));
         hipLaunchKernelGGL(k_pack_bf16, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->params + s->w_off[k],
                            s->wb[k], kCin[k], kCout[k], round16(9 * kCin[k]));
     }
     S_TRY(hipGetLastError());
     s->packed = true;
     return LBM_OK;
+}
+
+// FP32 convolution of x (Cin channels) with packed weights wm ([Cout16][round16(9 Cin)]) into y.
+void conv_f32(lbm_surrogate *s, const float *x, const float *wm, const float *bias, float *y, int Cin, int Cout, int l,
+              int B, bool relu)
+{
+    const int tx = (s->LW[l] + kMfmaCols - 1) / kMfmaCols, ty = (s->LH[l] + kMfmaRows - 1) / kMfmaRows;
+    const size_t lds = size_t(Cin) * (kMfmaRows + 2) * (kMfmaCols + 2) * sizeof(float);
+    hipLaunchKernelGGL(k_conv3x3_f32, dim3(unsigned(tx * ty), unsigned(B)), dim3(256), lds, s->stream, x, wm, bias, y,
+                       Cin, Cout, round16(9 * Cin), s->LH[l], s->LW[l], tx, relu ? 1 : 0);
 }
 
 void conv_forward_bf16(lbm_surrogate *s, const float *x, int k, float *y, int l, int B, bool relu)
@@ -620,17 +661,13 @@ void conv_forward(lbm_surrogate *s, const float *x, int k, float *y, int l, int 
         conv_forward_bf16(s, x, k, y, l, B, relu);
         return;
     }
-    const dim3 grid(unsigned(tiles(s, l)), unsigned(round16(kCout[k]) / kCoT), unsigned(B));
-    hipLaunchKernelGGL(k_conv3x3, grid, dim3(256), 0, s->stream, x, s->wp[k], s->params + s->b_off[k], y, kCin[k],
-                       kCout[k], round16(kCout[k]), s->LH[l], s->LW[l], tiles_x(s, l), relu ? 1 : 0);
+    conv_f32(s, x, s->wp[k], s->params + s->b_off[k], y, kCin[k], kCout[k], l, B, relu);
 }
 
 // dx = data gradient of conv k for output gradient g (dx written, not accumulated)
 void conv_dgrad(lbm_surrogate *s, const float *g, int k, float *dx, int l, int B)
 {
-    const dim3 grid(unsigned(tiles(s, l)), unsigned(round16(kCin[k]) / kCoT), unsigned(B));
-    hipLaunchKernelGGL(k_conv3x3, grid, dim3(256), 0, s->stream, g, s->wt[k], static_cast<const float *>(nullptr), dx,
-                       kCout[k], kCin[k], round16(kCin[k]), s->LH[l], s->LW[l], tiles_x(s, l), 0);
+    conv_f32(s, g, s->wt[k], nullptr, dx, kCout[k], kCin[k], l, B, false);
 }
 
 void conv_wgrad(lbm_surrogate *s, const float *x, const float *g, int k, int l, int B)
