@@ -34,10 +34,12 @@ def test_warm_start_needs_fewer_steps_than_rest():
     print("rest vs warm:", s, info, "ratio of means over all domains:", ratio_all)
     assert s["warm_unconverged"] <= s["rest_unconverged"], s
     assert s["both_converged"] >= 40
-    # measured on 1000 held-out domains: 1.32x fewer steps on average (median
-    # ratio 1.29); 1.36x on these 48
-    assert ratio_all >= 1.25, (ratio_all, s)
-    assert s["ratio_of_means"] >= 1.25, s
+    # measured on 1000 held-out domains: 1.38x fewer steps on average (median
+    # ratio 1.34); on these 48: 1.37x over all domains, 1.43x over the 46
+    # that both starts bring to 1e-6 (1.36x before the column-flux density
+    # profile of the warm start)
+    assert ratio_all >= 1.33, (ratio_all, s)
+    assert s["ratio_of_means"] >= 1.38, s
     assert s["warm_median"] < s["rest_median"]
 
 

@@ -66,15 +66,67 @@ def test_masked_smoothing_keeps_constants():
 
 def test_model_file_round_trip(tmp_path):
     p = np.arange(10, dtype=np.float32)
-    m = sg.Model(p, 256, 256, 39.5, 4.0, 0.5, 2.0, {"samples": 3})
+    m = sg.Model(p, 256, 256, 39.5, 4.0, 0.5, 2.0, {"samples": 3}, rho_profile=8.0)
     path = tmp_path / "m.npz"
     m.save(path)
     m2 = sg.Model.load(path)
     np.testing.assert_array_equal(m2.params, p)
-    assert (m2.height, m2.width, m2.j_scale, m2.rho_scale, m2.j_blend, m2.rho_sigma) == (256, 256, 39.5, 4.0, 0.5, 2.0)
+    assert (m2.height, m2.width, m2.j_scale, m2.rho_scale, m2.j_blend, m2.rho_sigma, m2.rho_profile) == \
+        (256, 256, 39.5, 4.0, 0.5, 2.0, 8.0)
     assert m2.meta == {"samples": 3}
     with np.load(path, allow_pickle=False) as z:  # nothing pickled
         assert set(z.files) == {"params", "shape", "encoding", "meta"}
+    # a file written before the rho_profile option loads with it off
+    np.savez(path, params=p, shape=np.array([256, 256]), encoding=np.array([39.5, 4.0, 0.5, 2.0]),
+             meta=np.array("{}"))
+    assert sg.Model.load(path).rho_profile == 0.0
+
+
+def _channel(nx=40, ny=12, rin=1.04, rout=1.0):
+    """All-fluid channel, column means falling linearly, one flux."""
+    x = np.arange(nx)
+    mean = rin - (rin - rout) * (x + 0.5) / nx
+    return np.ones((ny, nx), bool), np.tile(mean, (ny, 1)), np.full((ny, nx), 0.01)
+
+
+def test_darcy_profile_keeps_deviations_and_spans_the_drop():
+    mask, rho, jx = _channel()
+    dev = 0.001 * np.sin(np.arange(12))[:, None] * np.cos(np.arange(40))[None, :]
+    dev = dev - dev.mean(axis=0)
+    bad = 1.0 + 0.8 * (rho - 1.0) + 0.003  # the right shape, 80 % of the drop, offset
+    out = sg.darcy_profile(bad + dev, jx, mask, 1.04, 1.0, 4.0)
+    mean = out.mean(axis=0)
+    np.testing.assert_allclose(out - mean[None, :], dev, atol=1e-12)
+    assert np.all(np.diff(mean) < 0) and 1.0 < mean[-1] < mean[0] < 1.04
+    # about half a column's fall between each boundary density and its column
+    assert 0.25 < (1.04 - mean[0]) / (mean[0] - mean[1]) < 1.0
+    assert 0.25 < (mean[-1] - 1.0) / (mean[-2] - mean[-1]) < 1.0
+
+
+def test_darcy_profile_mirror_symmetry():
+    r = np.random.default_rng(5)
+    mask = r.random((16, 48)) < 0.8
+    rho = 1.0 + 0.03 * r.random((16, 48))
+    jx = 0.01 + 0.005 * r.random((16, 48))
+    a = sg.darcy_profile(rho, jx, mask, 1.03, 1.0, 3.0)
+    b = sg.darcy_profile((2.03 - rho)[:, ::-1], jx[:, ::-1], mask[:, ::-1], 1.03, 1.0, 3.0)
+    np.testing.assert_allclose(np.where(mask, a, 0), np.where(mask, 2.03 - b[:, ::-1], 0), atol=1e-12)
+
+
+def test_darcy_profile_follows_the_flux():
+    """Where the predicted flux is twice as large for the same fall the column
+    resistance halves, so the rebuilt profile falls half as steeply there; a
+    column without fluid takes interpolated values and stays solid."""
+    mask, rho, jx = _channel()
+    jx[:, 20:] *= 2.0
+    mask[:, 5] = False
+    out = sg.darcy_profile(rho, jx, mask, 1.04, 1.0, 1e-3)
+    mean = out[0]
+    steep = np.diff(mean[2:18]).mean()
+    flat = np.diff(mean[22:38]).mean()
+    assert flat == pytest.approx(steep / 2.0, rel=1e-6)
+    assert np.all(out[:, 5] == rho[:, 5])
+    assert np.all(np.diff(mean[np.r_[0:5, 6:40]]) < 0)
 
 
 def test_porous_docs_ranges():
