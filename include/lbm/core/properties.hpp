@@ -23,6 +23,9 @@ struct Extensions
     std::string storage = "f64";
     // rounding of the FP16/BF16 population stores: "nearest" or "stochastic"
     std::string rounding = "nearest";
+    // npol/nsol step kernel: "auto", "sweep" (one wave per tile) or "block"
+    // (one work-group per tile, buffer refresh launches)
+    std::string tile_kernel = "auto";
     // outputs that are NaN or above FLT_MAX are written as 0 (the reference's
     // WITH_NAN_PROTECTION build option, ON by default)
     bool nan_protection = true;

@@ -86,6 +86,17 @@ enum lbm_storage {
  * every step near a steady state.  No counterpart in the reference. */
 enum lbm_rounding { LBM_ROUND_NEAREST = 0, LBM_ROUND_STOCHASTIC = 1 };
 
+/* Step kernel of npol/nsol lattices whose tiles are at most 64 nodes wide.
+ * Sweep: one wave walks each tile, the tile-edge copies in side arrays (no
+ * refresh launches); block: one work-group per tile with the reference's
+ * buffer refresh before each step.  Auto: the sweep when the lattice has at
+ * least two tiles per compute unit and the layout is not npol's FP64
+ * collision layout, else the block kernel (a lattice of few tiles leaves
+ * most of the GPU idle under the sweep: 256 x 64 with 32 x 32 tiles, 16
+ * tiles, measured 25.7 against 9.1 us per step).  Both give the same
+ * fields.  No counterpart in the reference. */
+enum lbm_tile_kernel { LBM_TILE_AUTO = 0, LBM_TILE_SWEEP = 1, LBM_TILE_BLOCK = 2 };
+
 /* Node-code bits returned by lbm_get_node_codes (one uint16 per expanded node).
  * Bit d (0..8) set: the population pulled along direction d comes from a
  * solid node, so half-way bounce-back supplies the node's own post-collision
@@ -116,6 +127,7 @@ typedef struct lbm_desc {
     uint32_t subdomain_count_vertical;
     uint32_t work_group_size;
     uint32_t rounding;      /* enum lbm_rounding: stores of the FP16/BF16 forms in the step kernels */
+    uint32_t tile_kernel;   /* enum lbm_tile_kernel (npol/nsol) */
     double relaxation_time;
     double inlet_density;
     double inlet_velocity_x;

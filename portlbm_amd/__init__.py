@@ -28,6 +28,7 @@ ALGORITHMS = {"nptl": 0, "lptl": 1, "npol": 2, "nsol": 3}
 LAYOUTS = {"collision": 0, "stream": 1, "bundle": 2}
 STORAGES = {"f64": 0, "f32": 1, "f16-shifted": 2, "bf16-shifted": 3}
 ROUNDINGS = {"nearest": 0, "stochastic": 1}
+TILE_KERNELS = {"auto": 0, "sweep": 1, "block": 2}
 STORAGE_BYTES = {"f64": 8, "f32": 4, "f16-shifted": 2, "bf16-shifted": 2}
 
 CODE_BOUNCE_MASK = 0x01FF
@@ -63,6 +64,7 @@ class LbmDesc(ctypes.Structure):
         ("subdomain_count_vertical", ctypes.c_uint32),
         ("work_group_size", ctypes.c_uint32),
         ("rounding", ctypes.c_uint32),
+        ("tile_kernel", ctypes.c_uint32),
         ("relaxation_time", ctypes.c_double),
         ("inlet_density", ctypes.c_double),
         ("inlet_velocity_x", ctypes.c_double),
@@ -196,7 +198,7 @@ def settings(algorithm: str = "nptl", data_layout: str = "stream", *, inner_h: i
              rho_in: float = 1.2, rho_out: float = 1.0, u_in: tuple[float, float] = (0.0, 0.0),
              u_out: tuple[float, float] = (0.0, 0.0), **extensions: Any) -> dict:
     """A settings document; keyword extensions go to the "mi355x" section
-    (storage, rounding, nanProtection, seed, porosity, radiusMin, radiusMax,
+    (storage, rounding, tileKernel, nanProtection, seed, porosity, radiusMin, radiusMax,
     obstacleMargin, cylinderX, cylinderRadius)."""
     doc = {
         "algorithmic": {
@@ -274,6 +276,7 @@ def desc_from_settings(doc: dict) -> LbmDesc:
     d.storage = STORAGES[ext.get("storage", "f64")]
     d.nan_protection = 1 if ext.get("nanProtection", True) else 0
     d.rounding = ROUNDINGS[ext.get("rounding", "nearest")]
+    d.tile_kernel = TILE_KERNELS[ext.get("tileKernel", "auto")]
     d.horizontal_nodes, d.vertical_nodes = e["H"], e["V"]
     d.expanded_horizontal_nodes, d.expanded_vertical_nodes = e["Hx"], e["Vx"]
     d.subdomain_horizontal_nodes, d.subdomain_vertical_nodes = e["sub_h"], e["sub_v"]

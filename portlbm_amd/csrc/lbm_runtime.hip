@@ -1592,6 +1592,8 @@ static int validate_desc(const lbm_desc *d)
         return fail(LBM_ERR_INVALID_ARGUMENT, "unknown storage type");
     if (d->rounding != LBM_ROUND_NEAREST && d->rounding != LBM_ROUND_STOCHASTIC)
         return fail(LBM_ERR_INVALID_ARGUMENT, "unknown rounding mode");
+    if (d->tile_kernel > LBM_TILE_BLOCK)
+        return fail(LBM_ERR_INVALID_ARGUMENT, "unknown tile kernel");
     if (d->horizontal_nodes < 4 || d->vertical_nodes < 4)
         return fail(LBM_ERR_INVALID_ARGUMENT, "horizontal_nodes and vertical_nodes must be >= 4");
     if (!(d->relaxation_time > 0.0))
@@ -1745,9 +1747,24 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     // (npol's FP64 collision layout excepted: its strided 72-byte nodes ran
     // faster with the block-per-tile kernel and the refresh launches, 15.5
     // against 14.6 GFLUPS on the porous channel; nsol's ran 14.8 against 14.0)
+    // A lattice with fewer tiles than two per compute unit runs the block
+    // kernels under LBM_TILE_AUTO: the sweep gives each tile one wave, which
+    // then walks it row group by row group (npol stream FP64 per step:
+    // 256 x 64, 16 tiles, 25.7 us sweep against 9.1 us block; 1024 x 256,
+    // 248 tiles, 28.4 / 20.1; 2048 x 512, 992 tiles, 41.5 / 50.6; 4096 x
+    // 1024, 3844 tiles, 154 / 174).
     size_t side_bytes = 0;
-    if (c->buffered && c->sub_h <= TILE_SWEEP_MAX_WIDTH &&
-        !(desc->algorithm == LBM_ALG_NPOL && desc->layout == LBM_LAYOUT_COLLISION && c->elem == 8))
+    int cus = 0;
+    {
+        int dev = 0;
+        CREATE_TRY(hipGetDevice(&dev));
+        CREATE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const bool few_tiles = uint64_t(c->cnt_h) * c->cnt_v < 2ull * uint64_t(std::max(cus, 1));
+    const bool auto_sweep =
+        !few_tiles && !(desc->algorithm == LBM_ALG_NPOL && desc->layout == LBM_LAYOUT_COLLISION && c->elem == 8);
+    if (c->buffered && c->sub_h <= TILE_SWEEP_MAX_WIDTH && desc->tile_kernel != LBM_TILE_BLOCK &&
+        (desc->tile_kernel == LBM_TILE_SWEEP || auto_sweep))
     {
         const size_t align = 256 / c->elem;
         auto round_up = [&](size_t n) { return (n + align - 1) / align * align; };

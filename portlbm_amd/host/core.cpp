@@ -27,7 +27,8 @@ namespace core
 bool Extensions::is_default() const
 {
     const Extensions d;
-    return storage == d.storage && rounding == d.rounding && nan_protection == d.nan_protection && seed == d.seed && porosity == d.porosity &&
+    return storage == d.storage && rounding == d.rounding && tile_kernel == d.tile_kernel &&
+           nan_protection == d.nan_protection && seed == d.seed && porosity == d.porosity &&
            radius_min == d.radius_min && radius_max == d.radius_max && obstacle_margin == d.obstacle_margin &&
            max_attempts == d.max_attempts &&
            cylinder_x == d.cylinder_x && cylinder_radius == d.cylinder_radius;
@@ -107,6 +108,9 @@ void Properties::validate() const
     if (extensions.rounding != "nearest" && extensions.rounding != "stochastic")
         throw PropertyArgumentException("Unknown rounding: \"" + extensions.rounding +
                                         "\". Valid values: nearest, stochastic");
+    if (extensions.tile_kernel != "auto" && extensions.tile_kernel != "sweep" && extensions.tile_kernel != "block")
+        throw PropertyArgumentException("Unknown tile kernel: \"" + extensions.tile_kernel +
+                                        "\". Valid values: auto, sweep, block");
     if (!(extensions.porosity > 0.0 && extensions.porosity <= 1.0))
         throw PropertyArgumentException("porosity must be in (0, 1]");
     if (!(extensions.radius_min > 0.0 && extensions.radius_max >= extensions.radius_min))

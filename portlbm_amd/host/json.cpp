@@ -530,6 +530,7 @@ JsonValue properties_to_json_value(const core::Properties &p)
         JsonValue &x = doc["mi355x"];
         x["storage"] = JsonValue::string(e.storage);
         x["rounding"] = JsonValue::string(e.rounding);
+        x["tileKernel"] = JsonValue::string(e.tile_kernel);
         x["nanProtection"] = JsonValue::boolean(e.nan_protection);
         x["seed"] = JsonValue::unsigned_number(e.seed);
         x["porosity"] = JsonValue::number(e.porosity);
@@ -581,6 +582,8 @@ core::Properties json_value_to_properties(const JsonValue &data, int offset)
             e.storage = f->as_string("mi355x.storage");
         if (const JsonValue *f = x->find("rounding"))
             e.rounding = f->as_string("mi355x.rounding");
+        if (const JsonValue *f = x->find("tileKernel"))
+            e.tile_kernel = f->as_string("mi355x.tileKernel");
         if (const JsonValue *f = x->find("nanProtection"))
             e.nan_protection = f->as_bool("mi355x.nanProtection");
         if (const JsonValue *f = x->find("seed"))

@@ -24,9 +24,9 @@ from portlbm_amd import workloads
 pytestmark = pytest.mark.gpu
 
 
-def _fields(alg, layout, storage, wg, steps=300, w=130, h=43):
+def _fields(alg, layout, storage, wg, steps=300, w=130, h=43, tile="auto"):
     doc = pa.settings(alg, layout, inner_h=w, inner_v=h, work_group_size=wg, time_steps=steps, scenario="circle",
-                      tau=0.7, rho_in=1.05, rho_out=1.0, storage=storage, rounding="nearest")
+                      tau=0.7, rho_in=1.05, rho_out=1.0, storage=storage, rounding="nearest", tileKernel=tile)
     with pa.Lattice.from_settings(doc) as lat:
         lat.run(steps)
         return lat.macros()
@@ -44,9 +44,10 @@ CASES = [("lptl", 256), ("npol", 256), ("npol", 1024), ("npol", 130), ("nsol", 2
 def test_sixteen_bit_fields_equal_across_algorithms(storage, layout):
     ref = _fields("nptl", layout, storage, 256)
     for alg, wg in CASES:
-        got = _fields(alg, layout, storage, wg)
-        for name, a, b in zip(("rho", "jx", "jy", "|j|"), got, ref):
-            assert np.array_equal(a, b), (alg, wg, layout, storage, name, float(np.max(np.abs(a - b))))
+        for tile in (("sweep", "block") if alg in ("npol", "nsol") else ("auto",)):
+            got = _fields(alg, layout, storage, wg, tile=tile)
+            for name, a, b in zip(("rho", "jx", "jy", "|j|"), got, ref):
+                assert np.array_equal(a, b), (alg, wg, tile, layout, storage, name, float(np.max(np.abs(a - b))))
 
 
 # Odd lattices for the column reference: tiny and ragged inner sizes,
@@ -67,24 +68,28 @@ def _outlet_quirk(alg, w, h, wg):
     return alg in ("npol", "nsol") and (d["H"] - 3) % d["sub_h"] == 0
 
 
+RAGGED16_KERNELS = [(a, w, h, wg, t) for a, w, h, wg in RAGGED16
+                    for t in (("sweep", "block") if a in ("npol", "nsol") else ("auto",))]
+
+
 @pytest.mark.parametrize("layout", ["collision", "stream", "bundle"])
-@pytest.mark.parametrize("alg,w,h,wg", RAGGED16)
-def test_sixteen_bit_ragged_shapes(alg, w, h, wg, layout):
+@pytest.mark.parametrize("alg,w,h,wg,tile", RAGGED16_KERNELS)
+def test_sixteen_bit_ragged_shapes(alg, w, h, wg, tile, layout):
     """FP16, round-to-nearest, 300 steps (two reference refreshes), finite
     fields.  Lattices without the outlet quirk equal nptl's on the same inner
     lattice bit for bit; those with it stay within FP16's distance of the
     same algorithm in FP64 (measured below 2e-6 in density and 4e-4 in
     momentum, relative L2)."""
-    got = _fields(alg, layout, "f16-shifted", wg, 300, w, h)
+    got = _fields(alg, layout, "f16-shifted", wg, 300, w, h, tile)
     for name, a in zip(("rho", "jx", "jy", "|j|"), got):
         assert np.all(np.isfinite(a)), (alg, w, h, wg, layout, name)
     if _outlet_quirk(alg, w, h, wg):
-        ref = _fields(alg, layout, "f64", wg, 300, w, h)
+        ref = _fields(alg, layout, "f64", wg, 300, w, h, tile)
         fluid = ref[0] > -1
         e_rho = float(np.linalg.norm(got[0][fluid] - ref[0][fluid]) / np.linalg.norm(ref[0][fluid]))
         e_j = float(np.linalg.norm(np.hypot(got[1] - ref[1], got[2] - ref[2])[fluid])
                     / np.linalg.norm(np.hypot(ref[1], ref[2])[fluid]))
-        print(alg, w, h, wg, layout, "outlet quirk: FP16 vs FP64 rho", e_rho, "j", e_j, flush=True)
+        print(alg, w, h, wg, tile, layout, "outlet quirk: FP16 vs FP64 rho", e_rho, "j", e_j, flush=True)
         assert e_rho <= 6e-6 and e_j <= 1.2e-3, (e_rho, e_j)
         return
     key = (w, h, layout)

@@ -14,26 +14,29 @@ def _doc(alg, layout, storage, steps, rounding="nearest", **kw):
                        frame_update_interval=50, tau=0.7, rho_in=1.05, storage=storage, rounding=rounding, **kw)
 
 
-@pytest.mark.parametrize("alg,layout,storage,rounding", [
-    ("nptl", "stream", "f64", "nearest"), ("lptl", "bundle", "f32", "nearest"),
-    ("npol", "collision", "f64", "nearest"), ("nsol", "stream", "f16-shifted", "stochastic"),
-    pytest.param("nptl", "collision", "bf16-shifted", "nearest", marks=pytest.mark.skip(
+@pytest.mark.parametrize("alg,layout,storage,rounding,tile", [
+    ("nptl", "stream", "f64", "nearest", "auto"), ("lptl", "bundle", "f32", "nearest", "auto"),
+    ("npol", "collision", "f64", "nearest", "auto"), ("nsol", "stream", "f16-shifted", "stochastic", "auto"),
+    # the one-wave sweep and its side arrays (re-primed from the loaded lattice)
+    ("npol", "collision", "f64", "nearest", "sweep"), ("nsol", "stream", "f16-shifted", "stochastic", "sweep"),
+    pytest.param("nptl", "collision", "bf16-shifted", "nearest", "auto", marks=pytest.mark.skip(
         reason="one run ended in a host segmentation fault inside portlbm_destroy as its first handler "
                "closed (no GPU fault; the case passed in every earlier full run); the cause is not yet "
                "shown, so the case stays off the GPU"))])
-def test_restart_continues_bit_for_bit(tmp_path, alg, layout, storage, rounding):
-    with pa.Handler(_doc(alg, layout, storage, 300, rounding)) as ref:
+def test_restart_continues_bit_for_bit(tmp_path, alg, layout, storage, rounding, tile):
+    kw = {"tileKernel": tile}
+    with pa.Handler(_doc(alg, layout, storage, 300, rounding, **kw)) as ref:
         ref.start()
         ref.block_until_finished()
         expect = [ref.field(n) for n in ("density", "x_velocity", "y_velocity", "absolute_velocity")]
         expect_pop = ref.populations()
     snap = tmp_path / "s.plbm"
-    with pa.Handler(_doc(alg, layout, storage, 123, rounding)) as first:
+    with pa.Handler(_doc(alg, layout, storage, 123, rounding, **kw)) as first:
         first.start()
         first.block_until_finished()
         first.save_snapshot(snap)
         at_save = first.field("density")
-    with pa.Handler(_doc(alg, layout, storage, 300, rounding)) as second:
+    with pa.Handler(_doc(alg, layout, storage, 300, rounding, **kw)) as second:
         second.load_snapshot(snap)
         assert second.get_current_iteration() == 123
         np.testing.assert_array_equal(second.field("density"), at_save)  # results restored too

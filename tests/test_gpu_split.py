@@ -15,13 +15,13 @@ CY = np.array([d // 3 - 1 for d in range(9)], dtype=float)
 W = np.array([1 / 36, 1 / 9, 1 / 36, 1 / 9, 4 / 9, 1 / 9, 1 / 36, 1 / 9, 1 / 36])
 
 
-def _doc(alg, layout, storage="f64", rounding="nearest", scenario="circle"):
+def _doc(alg, layout, storage="f64", rounding="nearest", scenario="circle", tile="auto"):
     wg = 64 if alg in ("npol", "nsol") else 128
     return pa.settings(alg, layout, inner_h=80, inner_v=36, scenario=scenario, work_group_size=wg, time_steps=1,
-                       tau=0.7, rho_in=1.05, storage=storage, rounding=rounding)
+                       tau=0.7, rho_in=1.05, storage=storage, rounding=rounding, tileKernel=tile)
 
 
-@pytest.mark.parametrize("alg,layout,storage,rounding", [
+SPLIT_CASES = [
     ("nptl", "stream", "f64", "nearest"), ("nptl", "collision", "f32", "nearest"),
     ("lptl", "bundle", "f64", "nearest"), ("npol", "stream", "f64", "nearest"),
     ("nsol", "collision", "f64", "nearest"), ("nsol", "bundle", "f16-shifted", "stochastic"),
@@ -29,9 +29,15 @@ def _doc(alg, layout, storage="f64", rounding="nearest", scenario="circle"):
     ("nptl", "collision", "f16-shifted", "stochastic"), ("lptl", "collision", "bf16-shifted", "nearest"),
     ("nptl", "bundle", "f32", "nearest"), ("lptl", "bundle", "f16-shifted", "nearest"),
     ("nptl", "stream", "f16-shifted", "stochastic"), ("npol", "bundle", "f32", "nearest"),
-    ("nsol", "stream", "f32", "nearest")])
-def test_split_steps_equal_fused_steps(alg, layout, storage, rounding):
-    doc = _doc(alg, layout, storage, rounding)
+    ("nsol", "stream", "f32", "nearest")]
+
+
+# npol/nsol with both step kernels of narrow tiles (the sweep and the block kernel)
+@pytest.mark.parametrize("alg,layout,storage,rounding,tile",
+                         [c + (t,) for c in SPLIT_CASES
+                          for t in (("sweep", "block") if c[0] in ("npol", "nsol") else ("auto",))])
+def test_split_steps_equal_fused_steps(alg, layout, storage, rounding, tile):
+    doc = _doc(alg, layout, storage, rounding, tile=tile)
     with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
         # fluid slots: solid slots (and the npol/nsol buffer copies of them) are never read, and
         # the fused kernels zero some of them by wave segments
@@ -79,7 +85,7 @@ def test_staged_kernels_equal_split_steps_on_ragged_shapes(alg, layout, storage,
     ("npol", 97, 40, 256, "stream", "f16-shifted", "nearest")])
 def test_tile_sweep_equals_split_steps_on_multi_group_tiles(alg, w, h, wg, layout, storage, rounding):
     doc = pa.settings(alg, layout, inner_h=w, inner_v=h, scenario="circle", work_group_size=wg, time_steps=1,
-                      tau=0.7, rho_in=1.05, storage=storage, rounding=rounding)
+                      tau=0.7, rho_in=1.05, storage=storage, rounding=rounding, tileKernel="sweep")
     with pa.Lattice.from_settings(doc) as fused, pa.Lattice.from_settings(doc) as split:
         keep = _fluid_slots(fused)
         fused.run(41)
