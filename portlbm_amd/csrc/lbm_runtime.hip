@@ -207,11 +207,12 @@ constexpr bool nptl_staged()
 }
 
 // How the source ranges reach LDS: "flat" makes every 16-byte vector of every
-// range one work item and issues all loads of a thread before its LDS writes
-// (one wait for HBM per block); otherwise the block copies range by range.
-// Flat measured faster for the collision layout in FP32 (45.5 -> 49.9 GFLUPS)
-// and 16-bit (65.0 -> 66.9), slower for FP64 collision (21.2 -> 11.8) and
-// FP64 bundle (28.1 -> 20.3).
+// range one work item; otherwise thread i takes vector i of each range.
+// Both issue all loads of a thread before its LDS writes (one wait for HBM
+// per block).  Flat measured faster for the collision layout in FP32 (45.5
+// -> 49.9 GFLUPS against copying range by range) and 16-bit (65.0 -> 66.9),
+// slower for FP64 collision (21.2 -> 11.8) and FP64 bundle (28.1 -> 20.3),
+// where one vector per thread and range gives 31.9 and 31.8.
 template <int L, int S>
 constexpr bool nptl_staged_flat()
 {
@@ -299,18 +300,39 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
     }
     else
     {
-        // range by range, all threads on one range at a time
+        // Vector i of every range to thread i (mod NT); a thread issues the
+        // loads of all its ranges before any LDS write, so the block waits
+        // for HBM once (range by range it waited NR times: FP64 collision
+        // 23.2 -> 31.9 GFLUPS, FP64 bundle 30.0 -> 31.8 on the porous
+        // channel).
+        constexpr uint32_t NVR = (K * (B + 2u) + 2u * (VE - 1u)) / VE;
+        constexpr uint32_t KV = (NVR + NT - 1u) / NT;
+        V4 tmp[NR][KV];
 #pragma unroll
         for (uint32_t r = 0; r < NR; ++r)
         {
-            if (r < staged)
+            const uint32_t ea = range_start(r) - shift[r];
+            const uint32_t nvec = (shift[r] + K * (len + 2u) + VE - 1u) / VE;
+            const V4 *g = reinterpret_cast<const V4 *>(src + ea);
+#pragma unroll
+            for (uint32_t k = 0; k < KV; ++k)
             {
-                const uint32_t ea = range_start(r) - shift[r];
-                const uint32_t nvec = (shift[r] + K * (len + 2u) + VE - 1u) / VE;
-                const V4 *g = reinterpret_cast<const V4 *>(src + ea);
-                V4 *l = reinterpret_cast<V4 *>(stage + r * ROW);
-                for (uint32_t i = tid; i < nvec; i += NT)
-                    l[i] = g[i];
+                const uint32_t i = tid + k * NT;
+                if (r < staged && i < nvec)
+                    tmp[r][k] = g[i];
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < NR; ++r)
+        {
+            const uint32_t nvec = (shift[r] + K * (len + 2u) + VE - 1u) / VE;
+            V4 *l = reinterpret_cast<V4 *>(stage + r * ROW);
+#pragma unroll
+            for (uint32_t k = 0; k < KV; ++k)
+            {
+                const uint32_t i = tid + k * NT;
+                if (r < staged && i < nvec)
+                    l[i] = tmp[r][k];
             }
         }
     }
