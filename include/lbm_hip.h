@@ -89,13 +89,18 @@ enum lbm_rounding { LBM_ROUND_NEAREST = 0, LBM_ROUND_STOCHASTIC = 1 };
 /* Step kernel of npol/nsol lattices whose tiles are at most 64 nodes wide.
  * Sweep: one wave walks each tile, the tile-edge copies in side arrays (no
  * refresh launches); block: one work-group per tile with the reference's
- * buffer refresh before each step.  Auto: the sweep when the lattice has at
- * least two tiles per compute unit and the layout is not npol's FP64
- * collision layout, else the block kernel (a lattice of few tiles leaves
- * most of the GPU idle under the sweep: 256 x 64 with 32 x 32 tiles, 16
- * tiles, measured 25.7 against 9.1 us per step).  Both give the same
- * fields.  No counterpart in the reference. */
-enum lbm_tile_kernel { LBM_TILE_AUTO = 0, LBM_TILE_SWEEP = 1, LBM_TILE_BLOCK = 2 };
+ * buffer refresh before each step; staged (collision layout, tiles at least
+ * 8 nodes wide; other lattices run the automatic choice): one work-group per
+ * tile walks it a chunk of rows at a time through LDS, with the sweep's
+ * side arrays.  Auto: in the FP64 and FP32 collision layout the staged
+ * kernel on lattices of at least half a tile (FP64) or one tile (FP32) per
+ * compute unit; else the sweep when the lattice has at least two tiles per
+ * compute unit and the layout is not npol's FP64 collision layout; else the
+ * block kernel (a lattice of few tiles leaves most of the GPU idle under the
+ * sweep: 256 x 64 with 32 x 32 tiles, 16 tiles, measured 25.7 against 9.1
+ * us per step).  All give the same fields.  No counterpart in the
+ * reference. */
+enum lbm_tile_kernel { LBM_TILE_AUTO = 0, LBM_TILE_SWEEP = 1, LBM_TILE_BLOCK = 2, LBM_TILE_STAGED = 3 };
 
 /* Node-code bits returned by lbm_get_node_codes (one uint16 per expanded node).
  * Bit d (0..8) set: the population pulled along direction d comes from a

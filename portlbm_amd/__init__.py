@@ -28,7 +28,7 @@ ALGORITHMS = {"nptl": 0, "lptl": 1, "npol": 2, "nsol": 3}
 LAYOUTS = {"collision": 0, "stream": 1, "bundle": 2}
 STORAGES = {"f64": 0, "f32": 1, "f16-shifted": 2, "bf16-shifted": 3}
 ROUNDINGS = {"nearest": 0, "stochastic": 1}
-TILE_KERNELS = {"auto": 0, "sweep": 1, "block": 2}
+TILE_KERNELS = {"auto": 0, "sweep": 1, "block": 2, "staged": 3}
 STORAGE_BYTES = {"f64": 8, "f32": 4, "f16-shifted": 2, "bf16-shifted": 2}
 
 CODE_BOUNCE_MASK = 0x01FF

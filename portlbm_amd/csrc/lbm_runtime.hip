@@ -61,6 +61,8 @@ struct lbm_ctx
     // npol/nsol tiles at most one wave wide run k_step_tile_sweep, which keeps
     // the buffer copies in side arrays (two of each kind, used in turn)
     bool sweep = false;
+    // npol/nsol in the collision layout, tiles 8 to 64 nodes wide: k_step_tile_staged
+    bool tile_staged = false;
     uint32_t side_rows[2] = {0, 0};  // element offsets from lat[0]
     uint32_t side_cols[2] = {0, 0};
     uint32_t side_parity = 0;
@@ -209,10 +211,10 @@ constexpr bool nptl_staged()
 // How the source ranges reach LDS: "flat" makes every 16-byte vector of every
 // range one work item; otherwise thread i takes vector i of each range.
 // Both issue all loads of a thread before its LDS writes (one wait for HBM
-// per block).  Flat measured faster for the collision layout in FP32 (45.5
-// -> 49.9 GFLUPS against copying range by range) and 16-bit (65.0 -> 66.9),
-// slower for FP64 collision (21.2 -> 11.8) and FP64 bundle (28.1 -> 20.3),
-// where one vector per thread and range gives 31.9 and 31.8.
+// per block).  Flat measured faster for the collision layout in FP32 (26.9
+// -> 49.5 GFLUPS against one vector per thread and range), FP16 (59.1 ->
+// 61.6) and BF16 (56.2 -> 58.2), slower for FP64 collision (31.9 -> 11.8)
+// and FP64 bundle (31.8 -> 20.3).
 template <int L, int S>
 constexpr bool nptl_staged_flat()
 {
@@ -615,6 +617,7 @@ __global__ __launch_bounds__(1024) void k_step_nsol(StepParams p)
 //         j < 3 population 3j + 2 of (xb - 1, y), j >= 3 population 3(j - 3) of (xb + 1, y)
 constexpr uint32_t TILE_SWEEP_BLOCK = 256;  // four tiles per block
 constexpr uint32_t TILE_SWEEP_MAX_WIDTH = 64;
+constexpr uint32_t SWEEP_XCDS = 8;  // accelerator complex dies of an MI355X
 
 // Does the 64-byte segment of a lane's node hold a node of `pred`?  Lanes
 // lo..hi-1 hold consecutive places of one row; E elements per node are
@@ -652,7 +655,18 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
     constexpr uint32_t E = L == LBM_LAYOUT_STREAM ? 1u : (L == LBM_LAYOUT_BUNDLE ? 3u : 9u);
     constexpr uint32_t NL = sweep_nodes_per_lane<S>();
     T *lat = static_cast<T *>(p.dst);  // the side arrays follow the lattice in its allocation
-    const uint32_t tile = blockIdx.x * (TILE_SWEEP_BLOCK / 64u) + threadIdx.x / 64u;
+    // Blocks are dealt to the XCDs in turn.  In 16-bit storage each XCD takes
+    // a contiguous run of tiles instead, so that the segments and halo rows
+    // that neighbouring tiles share meet in one L2: on the porous 8192 x 2048
+    // channel (nsol stream) fetch -33 %, writes -10 %, time -2 %; in FP64
+    // the same mapping measured 1.4 % slower.
+    uint32_t blk = blockIdx.x;
+    if constexpr (sizeof(T) == 2)
+    {
+        const uint32_t nb = gridDim.x, q = nb / SWEEP_XCDS, rem = nb % SWEEP_XCDS, xcd = blockIdx.x % SWEEP_XCDS;
+        blk = xcd * q + min(xcd, rem) + blockIdx.x / SWEEP_XCDS;  // a permutation of 0 .. nb - 1
+    }
+    const uint32_t tile = blk * (TILE_SWEEP_BLOCK / 64u) + threadIdx.x / 64u;
     if (tile >= tiles)
         return;  // whole waves
     const uint32_t sh = p.sub_h, sv = p.sub_v, P = p.P, Sd = p.S, Hx = p.Hx, Vx = p.Vx;
@@ -820,6 +834,339 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
                         lat[p.side_rows_out + ((by - 1u) * 6u + j) * Hx + x] = t[j];
             }
         }
+    }
+}
+
+// npol and nsol in the collision layout, tiles 8 to 64 nodes wide: one block
+// per tile walks up it CR = 256 / sub_h node rows (a chunk) at a time
+// (npol_kernels.hpp:89-199, nsol_kernels.hpp:171-334 give the same fields:
+// every pull reads the previous step).  The tile's rows, with the buffer
+// (or ghost) rows and columns around it, pass through a ring of CR + 2 row
+// slots in LDS, each lattice row loaded once with aligned 16-byte loads.  A
+// chunk pulls from rows j - 1 .. j + 1 of the ring: its bottom neighbour row
+// has been overwritten in HBM by the previous chunk, but the ring still holds
+// its old values, and the rows above are not written yet.  The loads of the
+// next chunk's CR new rows are issued before the current chunk's update and
+// written to the ring slots of the rows it no longer needs after a barrier,
+// so HBM latency overlaps the update.  The buffer places of the staged rows
+// are then overwritten in the ring with the tile sweep's side arrays (what
+// the reference's refresh would have copied there; through a corner, the
+// diagonal neighbour's population), and the tile's edge nodes write their new
+// edge populations to the other side arrays, so no refresh launches are
+// needed.  Results leave through an LDS output area as 16-byte non-temporal
+// stores; a 64-byte segment of a tile row with no fluid node of the tile is
+// not written (solid places are never read); a chunk that holds a node that
+// is neither fluid nor solid (the outlet ghost column inside the last tile of
+// the buffered-outlet quirk) writes node by node and leaves that node alone.
+constexpr uint32_t TILE_STAGED_THREADS = 256;
+constexpr uint32_t TILE_STAGED_MIN_WIDTH = 8;
+constexpr uint32_t TILE_STAGED_SIDE = 2;  // side-array values a thread loads per chunk, at most
+
+// Nodes per thread and chunk (chunk rows CR = this x 256 / sub_h), by bytes
+// per population.
+constexpr uint32_t tile_staged_npt_of(size_t elem)
+{
+    return 1u;
+}
+template <int S>
+constexpr uint32_t tile_staged_npt()
+{
+    return tile_staged_npt_of(sizeof(typename Storage<S>::T));
+}
+
+// 16-byte vectors a thread loads per chunk, at most (sub_h 8 .. 64, the
+// first chunk's CR + 2 rows, rows offset by up to VE - 1 elements)
+template <int S>
+constexpr uint32_t tile_staged_vectors()
+{
+    constexpr uint32_t VE = 16u / sizeof(typename Storage<S>::T), NT = TILE_STAGED_THREADS;
+    uint32_t kv = 0;
+    for (uint32_t sh = TILE_STAGED_MIN_WIDTH; sh <= TILE_SWEEP_MAX_WIDTH; ++sh)
+    {
+        const uint32_t rows = tile_staged_npt<S>() * (NT / sh) + 2u;
+        const uint32_t nv = (VE - 1u + 9u * (sh + 2u) + VE - 1u) / VE;
+        const uint32_t k = (rows * nv + NT - 1u) / NT;
+        kv = k > kv ? k : kv;
+    }
+    return kv;
+}
+
+// side-array values of one chunk, at most: six per staged row (three for
+// each buffer column), 3 (sub_h + 2) for the buffer row below and above
+constexpr bool tile_staged_side_fits(uint32_t npt)
+{
+    for (uint32_t sh = TILE_STAGED_MIN_WIDTH; sh <= TILE_SWEEP_MAX_WIDTH; ++sh)
+        if (6u * (npt * (TILE_STAGED_THREADS / sh) + 2u) + 6u * (sh + 2u) > TILE_STAGED_SIDE * TILE_STAGED_THREADS)
+            return false;
+    return true;
+}
+static_assert(tile_staged_side_fits(tile_staged_npt_of(2)) && tile_staged_side_fits(tile_staged_npt_of(4)) &&
+                  tile_staged_side_fits(tile_staged_npt_of(8)),
+              "side values of a chunk exceed the per-thread slots");
+
+// Elements of a ring slot and of an output row, and the dynamic LDS bytes
+// (ring, output rows, two sets of fluid masks).
+__host__ __device__ inline uint32_t tile_staged_rowlen(uint32_t sh, uint32_t ve)
+{
+    return (9u * (sh + 2u) + 2u * ve - 2u) / ve * ve;
+}
+__host__ __device__ inline uint32_t tile_staged_outlen(uint32_t sh, uint32_t ve)
+{
+    return (9u * sh + 2u * ve - 2u) / ve * ve;
+}
+inline size_t tile_staged_lds(uint32_t sh, size_t elem)
+{
+    const uint32_t ve = uint32_t(16 / elem), cr = tile_staged_npt_of(elem) * (TILE_STAGED_THREADS / sh);
+    return (size_t(cr + 2u) * tile_staged_rowlen(sh, ve) + size_t(cr) * tile_staged_outlen(sh, ve)) * elem +
+           2u * size_t(cr) * sizeof(unsigned long long);
+}
+
+template <int S>
+__global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepParams p)
+{
+    using T = typename Storage<S>::T;
+    using C = typename Storage<S>::C;
+    using V4 = unsigned int __attribute__((ext_vector_type(4)));
+    constexpr int L = LBM_LAYOUT_COLLISION;
+    constexpr uint32_t NT = TILE_STAGED_THREADS, VE = 16u / sizeof(T), KV = tile_staged_vectors<S>();
+    constexpr uint32_t KS = TILE_STAGED_SIDE, NPT = tile_staged_npt<S>();
+    constexpr uint32_t SE = 64u / sizeof(T);  // elements per 64-byte segment
+    extern __shared__ __align__(16) unsigned char tile_lds[];
+    const uint32_t sh = p.sub_h, sv = p.sub_v, P = p.P, Hx = p.Hx, Vx = p.Vx;
+    const uint32_t R1 = NT / sh, CR = NPT * R1, W = CR + 2u;  // node rows per thread row pass, per chunk
+    const uint32_t rowlen = tile_staged_rowlen(sh, VE), outlen = tile_staged_outlen(sh, VE);
+    T *ring = reinterpret_cast<T *>(tile_lds);
+    T *out = ring + W * rowlen;
+    unsigned long long *masks = reinterpret_cast<unsigned long long *>(out + CR * outlen);  // [2][CR]
+    T *lat = static_cast<T *>(p.dst);
+    const uint32_t tid = threadIdx.x, ty = tid / sh, tx = tid - ty * sh;
+    const uint32_t bx = blockIdx.x, by = blockIdx.y;
+    const uint32_t x0 = bx * (sh + 1u) + 1u, y00 = by * (sv + 1u) + 1u;
+    const bool buf_l = bx > 0u, buf_r = bx + 1u < p.cnt_h;  // columns x0 - 1, x0 + sh are buffers
+    const bool buf_b = by > 0u, buf_t = by + 1u < p.cnt_v;  // rows y00 - 1, y00 + sv are buffers
+    // window row j (0 .. sv + 1) is lattice row y00 - 1 + j; its elements
+    // start at e00 + 9 j P, every row at the same offset mod VE (P is a
+    // multiple of 64 places); place i of a row is column x0 - 1 + i
+    const uint32_t e00 = 9u * place(x0 - 1u, y00 - 1u, P, p.X0);
+    const uint32_t shift = e00 % VE, a00 = e00 - shift;
+    const uint32_t nv = (shift + 9u * (sh + 2u) + VE - 1u) / VE;  // vectors per row
+    const uint32_t s1 = (e00 + 9u) % VE;                          // offset of the tile's first node mod VE
+    const uint32_t nvo = (s1 + 9u * sh + VE - 1u) / VE;           // output vectors per row
+    const uint32_t chunks = (sv + CR - 1u) / CR;
+    const uint32_t xe = x0 + tx, xu = xe - xe / (sh + 1u);
+    const Rt3 r3 = rt3_of<S>(p.ref_in, xu);
+    const float rto = rt_of<S>(p.ref_out, xu) - r3.own();
+
+    V4 pre[KV];
+    T side[KS];
+    uint32_t side_at[KS];  // ring element of each side value, ~0u: none
+    // loads of window rows ja .. jb, every lane the same count (branch-free
+    // addresses: lanes past the end load the first vector again), and of the
+    // side values of their buffer places
+    auto fetch = [&](uint32_t ja, uint32_t jb) {
+        const uint32_t total = (jb - ja + 1u) * nv;
+#pragma unroll
+        for (uint32_t k = 0; k < KV; ++k)
+        {
+            const uint32_t w = tid + k * NT;
+            const uint32_t wc = w < total ? w : 0u;
+            const uint32_t r = wc / nv, v = wc - r * nv;
+            pre[k] = *reinterpret_cast<const V4 *>(lat + (a00 + 9u * (ja + r) * P + v * VE));
+        }
+        // buffer-column places of rows that are not buffer rows: populations
+        // 2, 5, 8 on the left, 0, 3, 6 on the right; buffer rows: 6, 7, 8
+        // below the tile, 0, 1, 2 above it, at every place, a corner place
+        // from the column beyond the buffer column
+        const uint32_t ca = (ja == 0u && buf_b) ? 1u : ja, cb = (jb == sv + 1u && buf_t) ? sv : jb;
+        const uint32_t nc = 6u * (cb - ca + 1u);
+        const uint32_t nr = 3u * (sh + 2u);
+        const uint32_t nb = (ja == 0u && buf_b) ? nr : 0u, nt = (jb == sv + 1u && buf_t) ? nr : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < KS; ++k)
+        {
+            const uint32_t w = tid + k * NT;
+            side_at[k] = ~0u;
+            if (w < nc)
+            {
+                const uint32_t j = ca + w / 6u, s = (w % 6u) / 3u, q = w % 3u;
+                if (s == 0u ? buf_l : buf_r)
+                {
+                    const uint32_t kcol = s == 0u ? (bx - 1u) * 6u + q : bx * 6u + 3u + q;
+                    side[k] = lat[p.side_cols_in + kcol * Vx + (y00 - 1u + j)];
+                    side_at[k] = (j % W) * rowlen + shift + (s == 0u ? 3u * q + 2u : 9u * (sh + 1u) + 3u * q);
+                }
+            }
+            else if (w < nc + nb + nt)
+            {
+                const uint32_t u = w - nc, s = u < nb ? 0u : 1u, v = u - s * nb;
+                const uint32_t q = v / (sh + 2u), i = v - q * (sh + 2u);
+                const uint32_t xs = (i == 0u && buf_l) ? x0 - 2u : ((i == sh + 1u && buf_r) ? x0 + sh + 1u : x0 - 1u + i);
+                const uint32_t krow = s == 0u ? (by - 1u) * 6u + 3u + q : by * 6u + q;
+                const uint32_t j = s == 0u ? 0u : sv + 1u;
+                side[k] = lat[p.side_rows_in + krow * Hx + xs];
+                side_at[k] = (j % W) * rowlen + shift + 9u * i + (s == 0u ? 6u + q : q);
+            }
+        }
+    };
+    auto stash = [&](uint32_t ja, uint32_t jb) {
+        const uint32_t total = (jb - ja + 1u) * nv;
+#pragma unroll
+        for (uint32_t k = 0; k < KV; ++k)
+        {
+            const uint32_t w = tid + k * NT;
+            if (w < total)
+            {
+                const uint32_t r = w / nv, v = w - r * nv;
+                reinterpret_cast<V4 *>(ring + ((ja + r) % W) * rowlen)[v] = pre[k];
+            }
+        }
+    };
+    auto patch = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < KS; ++k)
+            if (side_at[k] != ~0u)
+                ring[side_at[k]] = side[k];
+    };
+    // this thread's node h of chunk k is in window row 1 + k CR + ty + h R1
+    auto code_of = [&](uint32_t k, uint32_t h) -> uint32_t {
+        const uint32_t j = 1u + k * CR + ty + h * R1;
+        return (ty < R1 && j <= sv) ? uint32_t(p.code[place(xe, y00 - 1u + j, P, p.X0)]) : 0u;
+    };
+
+    uint32_t code[NPT];
+#pragma unroll
+    for (uint32_t h = 0; h < NPT; ++h)
+        code[h] = code_of(0, h);
+    fetch(0u, min(CR + 1u, sv + 1u));
+    if (tid < 2u * CR)
+        masks[tid] = 0ull;
+    stash(0u, min(CR + 1u, sv + 1u));
+    __syncthreads();
+    patch();
+    __syncthreads();
+    for (uint32_t k = 0; k < chunks; ++k)
+    {
+        const bool more = k + 1u < chunks;
+        const uint32_t na = (k + 1u) * CR + 2u, nb = min((k + 2u) * CR + 1u, sv + 1u);  // next chunk's new rows
+        uint32_t code_next[NPT] = {};
+        if (more)
+        {
+#pragma unroll
+            for (uint32_t h = 0; h < NPT; ++h)
+                code_next[h] = code_of(k + 1u, h);
+            fetch(na, nb);
+        }
+        unsigned long long *fluid_row = masks + (k & 1u) * CR;
+        bool any_odd = false;
+        T t[NPT][9];
+#pragma unroll
+        for (uint32_t h = 0; h < NPT; ++h)
+        {
+            const uint32_t rr = ty + h * R1;  // node row within the chunk
+            const uint32_t j = 1u + k * CR + rr;  // window row of this node
+            const bool active = ty < R1 && j <= sv;
+            const uint32_t y = y00 - 1u + j;
+            const uint32_t pn = place(xe, y, P, p.X0);
+            const uint32_t cd = code[h];
+            const bool fluid = cd & LBM_CODE_FLUID, solid = cd & LBM_CODE_SOLID;
+            any_odd |= active && !fluid && !solid;
+            C f[9] = {};
+            if (fluid)
+            {
+#pragma unroll
+                for (int d = 0; d < 9; ++d)
+                {
+                    const bool bb = (cd >> d) & 1u;
+                    const uint32_t jr = bb ? j : uint32_t(int32_t(j) - cy_of(d));
+                    const uint32_t i = bb ? tx + 1u : uint32_t(int32_t(tx) + 1 - cx_of(d));
+                    const int dd = bb ? 8 - d : d;
+                    f[d] = Storage<S>::load(ring + (jr % W) * rowlen + shift, 9u * i + uint32_t(dd), dd,
+                                            bb ? 0.f : r3.rel(cx_of(d)));
+                }
+                finish_node<L, S>(p, cd, xu, y - y / (sv + 1u), f, r3.own());
+                atomicOr(&fluid_row[rr], 1ull << tx);
+            }
+            node_bits<S>(f, pn, p.sr, t[h], rto);
+            if (fluid)
+            {
+                // the new edge populations the neighbouring tiles pull next step
+                if (tx == sh - 1u && buf_r)
+                    for (uint32_t q = 0; q < 3; ++q)
+                        lat[p.side_cols_out + (bx * 6u + q) * Vx + y] = t[h][3u * q + 2u];
+                if (tx == 0u && buf_l)
+                    for (uint32_t q = 0; q < 3; ++q)
+                        lat[p.side_cols_out + ((bx - 1u) * 6u + 3u + q) * Vx + y] = t[h][3u * q];
+                if (j == sv && buf_t)
+                    for (uint32_t q = 0; q < 3; ++q)
+                        lat[p.side_rows_out + (by * 6u + 3u + q) * Hx + xe] = t[h][6u + q];
+                if (j == 1u && buf_b)
+                    for (uint32_t q = 0; q < 3; ++q)
+                        lat[p.side_rows_out + ((by - 1u) * 6u + q) * Hx + xe] = t[h][q];
+            }
+        }
+        // every pull of the chunk is done: the ring slots of rows below j - 1
+        // may take the next chunk's rows
+        const bool odd = __syncthreads_or(any_odd);
+#pragma unroll
+        for (uint32_t h = 0; h < NPT; ++h)
+        {
+            const uint32_t rr = ty + h * R1, j = 1u + k * CR + rr;
+            if (!(ty < R1 && j <= sv))
+                continue;
+            if (odd)
+            {
+                const uint32_t cd = code[h];
+                if (cd & (LBM_CODE_FLUID | LBM_CODE_SOLID))
+                {
+                    const uint32_t pn = place(xe, y00 - 1u + j, P, p.X0);
+#pragma unroll
+                    for (int d = 0; d < 9; ++d)
+                        lat[9u * pn + uint32_t(d)] = t[h][d];
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int d = 0; d < 9; ++d)
+                    out[rr * outlen + s1 + 9u * tx + uint32_t(d)] = t[h][d];
+            }
+        }
+        __syncthreads();
+        if (!odd)
+        {
+            const uint32_t rows = min(CR, sv - k * CR);
+            for (uint32_t w = tid; w < rows * nvo; w += NT)
+            {
+                const uint32_t r = w / nvo, v = w - r * nvo;
+                const uint32_t g0 = e00 + 9u + 9u * (1u + k * CR + r) * P;  // the row's first tile element
+                const uint32_t ga = g0 - s1 + v * VE;                       // 16-byte aligned
+                const uint32_t seg = ga / SE * SE;
+                const uint32_t lo = max(seg, g0), hi = min(seg + SE, g0 + 9u * sh) - 1u;
+                const uint32_t n0 = (lo - g0) / 9u, n = (hi - g0) / 9u - n0 + 1u;  // n <= SE / 9 + 2 < 64
+                if (!((fluid_row[r] >> n0) & ((1ull << n) - 1ull)))
+                    continue;
+                const T *o = out + r * outlen + v * VE;
+                if (v * VE >= s1 && v * VE + VE <= s1 + 9u * sh)
+                    __builtin_nontemporal_store(*reinterpret_cast<const V4 *>(o), reinterpret_cast<V4 *>(lat + ga));
+                else
+                    for (uint32_t e = 0; e < VE; ++e)
+                        if (v * VE + e >= s1 && v * VE + e < s1 + 9u * sh)
+                            lat[ga + e] = o[e];
+            }
+        }
+        if (tid < CR)
+            masks[((k + 1u) & 1u) * CR + tid] = 0ull;  // the next chunk's masks (last read a chunk ago)
+        // the output stores go out before the wait for the next rows
+        if (more)
+        {
+            stash(na, nb);
+            __syncthreads();
+            patch();
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < NPT; ++h)
+            code[h] = code_next[h];
+        __syncthreads();
     }
 }
 
@@ -1317,6 +1664,16 @@ struct Impl
                                dim3(TILE_SWEEP_BLOCK), lds, c->stream, p, tiles);
             c->side_parity ^= 1;
         };
+        auto launch_tile_staged = [&]() {
+            if constexpr (L == LBM_LAYOUT_COLLISION)
+            {
+                if (!c->side_valid)
+                    prime(c);
+                hipLaunchKernelGGL((k_step_tile_staged<S>), dim3(c->cnt_h, c->cnt_v), dim3(TILE_STAGED_THREADS),
+                                   tile_staged_lds(c->sub_h, sizeof(T)), c->stream, p);
+                c->side_parity ^= 1;
+            }
+        };
         switch (c->desc.algorithm)
         {
         case LBM_ALG_NPTL:
@@ -1355,6 +1712,11 @@ struct Impl
                 launch_sweep();
                 break;
             }
+            if (c->tile_staged)
+            {
+                launch_tile_staged();
+                break;
+            }
             refresh(c);
             hipLaunchKernelGGL((k_step_npol<L, S>), dim3(c->cnt_h, c->cnt_v), dim3(c->sub_h, c->sub_v), 0,
                                c->stream, p);
@@ -1364,6 +1726,11 @@ struct Impl
             if (c->sweep)
             {
                 launch_sweep();
+                break;
+            }
+            if (c->tile_staged)
+            {
+                launch_tile_staged();
                 break;
             }
             refresh(c);
@@ -1614,7 +1981,7 @@ static int validate_desc(const lbm_desc *d)
         return fail(LBM_ERR_INVALID_ARGUMENT, "unknown storage type");
     if (d->rounding != LBM_ROUND_NEAREST && d->rounding != LBM_ROUND_STOCHASTIC)
         return fail(LBM_ERR_INVALID_ARGUMENT, "unknown rounding mode");
-    if (d->tile_kernel > LBM_TILE_BLOCK)
+    if (d->tile_kernel > LBM_TILE_STAGED)
         return fail(LBM_ERR_INVALID_ARGUMENT, "unknown tile kernel");
     if (d->horizontal_nodes < 4 || d->vertical_nodes < 4)
         return fail(LBM_ERR_INVALID_ARGUMENT, "horizontal_nodes and vertical_nodes must be >= 4");
@@ -1763,12 +2130,13 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     // aligned 16-byte vectors, up to 15 bytes past the last place of the top
     // ghost row
     const size_t lat_bytes = c->lattice_elems * c->elem;
-    // npol/nsol tiles at most one wave wide: the tile sweep's side arrays go
-    // after the lattice in the same allocation (32-bit element offsets), two
-    // of each kind, each starting on 256 bytes
-    // (npol's FP64 collision layout excepted: its strided 72-byte nodes ran
-    // faster with the block-per-tile kernel and the refresh launches, 15.5
-    // against 14.6 GFLUPS on the porous channel; nsol's ran 14.8 against 14.0)
+    // npol/nsol tiles at most one wave wide: the side arrays of the tile sweep
+    // and of the staged block kernel go after the lattice in the same
+    // allocation (32-bit element offsets), two of each kind, each starting on
+    // 256 bytes.  Where the staged kernel does not run, npol's FP64 collision
+    // layout keeps the block kernel with the refresh launches: its strided
+    // 72-byte nodes ran faster there than under the sweep, 15.5 against 14.6
+    // GFLUPS on the porous channel (nsol's 14.8 against 14.0).
     // A lattice with fewer tiles than two per compute unit runs the block
     // kernels under LBM_TILE_AUTO: the sweep gives each tile one wave, which
     // then walks it row group by row group (npol stream FP64 per step:
@@ -1783,10 +2151,24 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
         CREATE_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
     const bool few_tiles = uint64_t(c->cnt_h) * c->cnt_v < 2ull * uint64_t(std::max(cus, 1));
+    // the staged block kernel (collision layout, FP64 and FP32) once the
+    // lattice has enough tiles: each block walks its tile chunk after chunk,
+    // about 16 us on any lattice, which the block kernels beat on small ones
+    // (FP64 at 768 x 192, 144 tiles, 19.4 against 19.5 us; FP32 at 1024 x
+    // 256, 248 tiles, 17.7 against 18.6 us).  In 16-bit storage the sweep
+    // kept its place: staged 36.5 / 37.5 against 35.9 / 37.4 GFLUPS (nsol /
+    // npol FP16 on the porous 8192 x 2048 channel), its side arrays costing
+    // 2-byte scattered loads and stores.
+    const uint64_t tiles = uint64_t(c->cnt_h) * c->cnt_v;
+    const bool auto_staged = c->elem >= 4 && tiles >= uint64_t(std::max(cus, 1)) / (c->elem == 8 ? 2u : 1u);
+    c->tile_staged = c->buffered && desc->layout == LBM_LAYOUT_COLLISION && c->sub_h >= TILE_STAGED_MIN_WIDTH &&
+                     c->sub_h <= TILE_SWEEP_MAX_WIDTH &&
+                     (desc->tile_kernel == LBM_TILE_STAGED || (desc->tile_kernel == LBM_TILE_AUTO && auto_staged));
     const bool auto_sweep =
         !few_tiles && !(desc->algorithm == LBM_ALG_NPOL && desc->layout == LBM_LAYOUT_COLLISION && c->elem == 8);
-    if (c->buffered && c->sub_h <= TILE_SWEEP_MAX_WIDTH && desc->tile_kernel != LBM_TILE_BLOCK &&
-        (desc->tile_kernel == LBM_TILE_SWEEP || auto_sweep))
+    const bool want_sweep = !c->tile_staged && c->buffered && c->sub_h <= TILE_SWEEP_MAX_WIDTH &&
+                            desc->tile_kernel != LBM_TILE_BLOCK && (desc->tile_kernel == LBM_TILE_SWEEP || auto_sweep);
+    if (want_sweep || c->tile_staged)
     {
         const size_t align = 256 / c->elem;
         auto round_up = [&](size_t n) { return (n + align - 1) / align * align; };
@@ -1795,13 +2177,15 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
         const size_t first = round_up(c->lattice_elems + 64 / c->elem);
         if (first + 2 * (rows + cols) < (1ull << 32))
         {
-            c->sweep = true;
+            c->sweep = want_sweep;
             c->side_rows[0] = uint32_t(first);
             c->side_rows[1] = uint32_t(first + rows);
             c->side_cols[0] = uint32_t(first + 2 * rows);
             c->side_cols[1] = uint32_t(first + 2 * rows + cols);
             side_bytes = (first + 2 * (rows + cols)) * c->elem - lat_bytes;
         }
+        else
+            c->tile_staged = false;  // the block kernels with the reference's refresh
     }
     CREATE_TRY(hipMalloc(&c->lat[0], lat_bytes + std::max<size_t>(64, side_bytes)));
     if (c->two_lattice)

@@ -108,9 +108,10 @@ void Properties::validate() const
     if (extensions.rounding != "nearest" && extensions.rounding != "stochastic")
         throw PropertyArgumentException("Unknown rounding: \"" + extensions.rounding +
                                         "\". Valid values: nearest, stochastic");
-    if (extensions.tile_kernel != "auto" && extensions.tile_kernel != "sweep" && extensions.tile_kernel != "block")
+    if (extensions.tile_kernel != "auto" && extensions.tile_kernel != "sweep" && extensions.tile_kernel != "block" &&
+        extensions.tile_kernel != "staged")
         throw PropertyArgumentException("Unknown tile kernel: \"" + extensions.tile_kernel +
-                                        "\". Valid values: auto, sweep, block");
+                                        "\". Valid values: auto, sweep, block, staged");
     if (!(extensions.porosity > 0.0 && extensions.porosity <= 1.0))
         throw PropertyArgumentException("porosity must be in (0, 1]");
     if (!(extensions.radius_min > 0.0 && extensions.radius_max >= extensions.radius_min))

@@ -142,9 +142,10 @@ void HipAlgorithmHandler::initialize_algorithm()
     desc.storage = storage;
     desc.nan_protection = properties_->extensions.nan_protection ? 1 : 0;
     desc.rounding = properties_->extensions.rounding == "stochastic" ? LBM_ROUND_STOCHASTIC : LBM_ROUND_NEAREST;
-    desc.tile_kernel = properties_->extensions.tile_kernel == "sweep"   ? LBM_TILE_SWEEP
-                       : properties_->extensions.tile_kernel == "block" ? LBM_TILE_BLOCK
-                                                                        : LBM_TILE_AUTO;
+    desc.tile_kernel = properties_->extensions.tile_kernel == "sweep"    ? LBM_TILE_SWEEP
+                       : properties_->extensions.tile_kernel == "block"  ? LBM_TILE_BLOCK
+                       : properties_->extensions.tile_kernel == "staged" ? LBM_TILE_STAGED
+                                                                         : LBM_TILE_AUTO;
     desc.horizontal_nodes = properties_->horizontal_nodes;
     desc.vertical_nodes = properties_->vertical_nodes;
     desc.expanded_horizontal_nodes = domain_->horizontal_nodes;

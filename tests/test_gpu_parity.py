@@ -55,15 +55,23 @@ RAGGED = [
 ]
 
 
-# npol/nsol shapes run with both step kernels of tiles at most a wave wide
-# (the one-wave sweep and the block kernel with buffer refreshes; on these
-# small lattices the automatic choice is the block kernel)
-RAGGED_KERNELS = [(a, w, h, wg, t) for a, w, h, wg in RAGGED
-                  for t in (("sweep", "block") if a in ("npol", "nsol") else ("auto",))]
+# npol/nsol shapes run with every step kernel of tiles at most a wave wide:
+# the one-wave sweep, the block kernel with buffer refreshes and, in the
+# collision layout, the staged block kernel (tiles 8 to 64 nodes wide; the
+# others run the automatic choice there)
 
 
-@pytest.mark.parametrize("layout", LAYOUTS)
-@pytest.mark.parametrize("alg,w,h,wg,tile", RAGGED_KERNELS)
+def _kernels(alg, layout):
+    if alg not in ("npol", "nsol"):
+        return ("auto",)
+    return ("sweep", "block", "staged") if layout == "collision" else ("sweep", "block")
+
+
+RAGGED_KERNELS = [(layout, a, w, h, wg, t) for layout in LAYOUTS for a, w, h, wg in RAGGED
+                  for t in _kernels(a, layout)]
+
+
+@pytest.mark.parametrize("layout,alg,w,h,wg,tile", RAGGED_KERNELS)
 def test_ragged_shapes(alg, w, h, wg, tile, layout):
     doc = pa.settings(alg, layout, inner_h=w, inner_v=h, work_group_size=wg, time_steps=300,
                       scenario="circle", tau=0.7, rho_in=1.05, tileKernel=tile)
@@ -72,8 +80,7 @@ def test_ragged_shapes(alg, w, h, wg, tile, layout):
     assert r["ok"], r
 
 
-@pytest.mark.parametrize("layout", LAYOUTS)
-@pytest.mark.parametrize("alg,w,h,wg,tile", RAGGED_KERNELS)
+@pytest.mark.parametrize("layout,alg,w,h,wg,tile", RAGGED_KERNELS)
 def test_ragged_shapes_fp32(alg, w, h, wg, tile, layout):
     """The same shapes in FP32 storage against the oracle's float instance,
     at tau 0.52 (the smallest relaxation time of the measured workloads: the

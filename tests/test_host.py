@@ -47,7 +47,7 @@ def test_library_exports_every_declared_symbol(header, lib):
     ({"work_group_size": 2048}, r"work_group_size must be in \[1, 1024\]"),
     ({"storage": 7}, "unknown storage type"),
     ({"rounding": 2}, "unknown rounding mode"),
-    ({"tile_kernel": 3}, "unknown tile kernel"),
+    ({"tile_kernel": 4}, "unknown tile kernel"),
     ({"expanded_horizontal_nodes": 10}, "expanded extents smaller than the lattice"),
     # 9 x 40000 x 20000 populations do not fit 32-bit indices
     ({"horizontal_nodes": 40000, "vertical_nodes": 20000, "expanded_horizontal_nodes": 40000,
@@ -100,6 +100,8 @@ def test_tile_kernel_option():
     assert pa.roundtrip(pa.settings("npol", tileKernel="block"))["mi355x"]["tileKernel"] == "block"
     assert pa.roundtrip(pa.settings("nsol", storage="f32"))["mi355x"]["tileKernel"] == "auto"
     assert pa.desc_from_settings(pa.settings("npol", tileKernel="sweep")).tile_kernel == 1
+    assert pa.desc_from_settings(pa.settings("npol", tileKernel="staged")).tile_kernel == 3
+    assert pa.roundtrip(pa.settings("nsol", tileKernel="staged"))["mi355x"]["tileKernel"] == "staged"
     assert pa.desc_from_settings(pa.settings("npol")).tile_kernel == 0
     with pytest.raises(pa.LbmError) as e:
         pa.validate(pa.settings("npol", tileKernel="warp"))
