@@ -124,8 +124,9 @@ def measured_traffic(algorithm: str, layout: str, storage: str, scenario: str = 
 
 def kernel_of(algorithm: str, layout: str, storage: str) -> str:
     """The step kernel the library launches for a cell of the default
-    workload (work-group 1024 for npol / nsol: 32 x 32 and 30 x 31 tiles;
-    256 for lptl's 1-D kernel).
+    workload (work-group 1024 for npol / nsol: 32 x 32 and 30 x 31 tiles,
+    the staged block kernel in their FP64/FP32 collision layout, the tile
+    sweep elsewhere; 256 for lptl's 1-D kernel).
     lptl runs nptl's kernels where those are faster: the staged kernel in the
     collision layout and the FP64/FP32 bundle layout, the row kernel in
     16-bit storage; its own 1-D kernel only in the FP64/FP32 stream layout."""
@@ -136,8 +137,8 @@ def kernel_of(algorithm: str, layout: str, storage: str) -> str:
         if algorithm == "lptl" and not sixteen:
             return "k_step_lptl"
         return "k_step_nptl"
-    if algorithm == "npol" and layout == "collision" and storage == "f64":
-        return "k_step_npol"
+    if layout == "collision" and not sixteen:
+        return "k_step_tile_staged"
     return "k_step_tile_sweep"
 
 
