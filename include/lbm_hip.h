@@ -89,10 +89,10 @@ enum lbm_rounding { LBM_ROUND_NEAREST = 0, LBM_ROUND_STOCHASTIC = 1 };
 /* Step kernel of npol/nsol lattices whose tiles are at most 64 nodes wide.
  * Sweep: one wave walks each tile, the tile-edge copies in side arrays (no
  * refresh launches); block: one work-group per tile with the reference's
- * buffer refresh before each step; staged (collision layout, tiles at least
- * 8 nodes wide; other lattices run the automatic choice): one work-group per
- * tile walks it a chunk of rows at a time through LDS, with the sweep's
- * side arrays.  Auto: in the FP64 and FP32 collision layout the staged
+ * buffer refresh before each step; staged (tiles at least 8 nodes wide;
+ * narrower ones run the automatic choice): one work-group per tile walks it
+ * a chunk of rows at a time through LDS, with the sweep's side arrays.
+ * Auto: in the FP64 and FP32 collision layout the staged
  * kernel on lattices of at least half a tile (FP64) or one tile (FP32) per
  * compute unit; else the sweep when the lattice has at least two tiles per
  * compute unit and the layout is not npol's FP64 collision layout; else the

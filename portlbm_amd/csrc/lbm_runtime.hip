@@ -837,55 +837,53 @@ __global__ __launch_bounds__(TILE_SWEEP_BLOCK) void k_step_tile_sweep(StepParams
     }
 }
 
-// npol and nsol in the collision layout, tiles 8 to 64 nodes wide: one block
-// per tile walks up it CR = 256 / sub_h node rows (a chunk) at a time
-// (npol_kernels.hpp:89-199, nsol_kernels.hpp:171-334 give the same fields:
-// every pull reads the previous step).  The tile's rows, with the buffer
-// (or ghost) rows and columns around it, pass through a ring of CR + 2 row
-// slots in LDS, each lattice row loaded once with aligned 16-byte loads.  A
-// chunk pulls from rows j - 1 .. j + 1 of the ring: its bottom neighbour row
-// has been overwritten in HBM by the previous chunk, but the ring still holds
-// its old values, and the rows above are not written yet.  The loads of the
-// next chunk's CR new rows are issued before the current chunk's update and
-// written to the ring slots of the rows it no longer needs after a barrier,
-// so HBM latency overlaps the update.  The buffer places of the staged rows
-// are then overwritten in the ring with the tile sweep's side arrays (what
-// the reference's refresh would have copied there; through a corner, the
-// diagonal neighbour's population), and the tile's edge nodes write their new
-// edge populations to the other side arrays, so no refresh launches are
-// needed.  Results leave through an LDS output area as 16-byte non-temporal
-// stores; a 64-byte segment of a tile row with no fluid node of the tile is
-// not written (solid places are never read); a chunk that holds a node that
-// is neither fluid nor solid (the outlet ghost column inside the last tile of
-// the buffered-outlet quirk) writes node by node and leaves that node alone.
+// npol and nsol, tiles 8 to 64 nodes wide: one block per tile walks up it
+// CR = 256 / sub_h node rows (a chunk) at a time (npol_kernels.hpp:89-199,
+// nsol_kernels.hpp:171-334 give the same fields: every pull reads the
+// previous step).  The tile's rows, with the buffer (or ghost) rows and
+// columns around it, pass through a ring of CR + 2 row slots in LDS, each
+// lattice row loaded once with aligned 16-byte loads: a row is one range of
+// 9 (sub_h + 2) elements in the collision layout, nine ranges of sub_h + 2
+// (one per direction array) in the stream layout, three of 3 (sub_h + 2) in
+// the bundle layout.  A chunk pulls from rows j - 1 .. j + 1 of the ring: its
+// bottom neighbour row has been overwritten in HBM by the previous chunk, but
+// the ring still holds its old values, and the rows above are not written
+// yet.  The loads of the next chunk's CR new rows are issued before the
+// current chunk's update and written to the ring slots of the rows it no
+// longer needs after a barrier, so HBM latency overlaps the update.  The
+// buffer places of the staged rows are then overwritten in the ring with the
+// tile sweep's side arrays (what the reference's refresh would have copied
+// there; through a corner, the diagonal neighbour's population), and the
+// tile's edge nodes write their new edge populations to the other side
+// arrays, so no refresh launches are needed.  Results leave through an LDS
+// output area as 16-byte non-temporal stores; a 64-byte segment of a tile row
+// with no fluid node of the tile is not written (solid places are never
+// read); a chunk that holds a node that is neither fluid nor solid (the
+// outlet ghost column inside the last tile of the buffered-outlet quirk)
+// writes node by node and leaves that node alone.
 constexpr uint32_t TILE_STAGED_THREADS = 256;
 constexpr uint32_t TILE_STAGED_MIN_WIDTH = 8;
 constexpr uint32_t TILE_STAGED_SIDE = 2;  // side-array values a thread loads per chunk, at most
 
-// Nodes per thread and chunk (chunk rows CR = this x 256 / sub_h), by bytes
-// per population.
-constexpr uint32_t tile_staged_npt_of(size_t elem)
+// elements of one node in one range of a row: collision 9, stream 1, bundle 3
+__host__ __device__ constexpr uint32_t tile_staged_k(int L)
 {
-    return 1u;
-}
-template <int S>
-constexpr uint32_t tile_staged_npt()
-{
-    return tile_staged_npt_of(sizeof(typename Storage<S>::T));
+    return L == LBM_LAYOUT_COLLISION ? 9u : (L == LBM_LAYOUT_STREAM ? 1u : 3u);
 }
 
 // 16-byte vectors a thread loads per chunk, at most (sub_h 8 .. 64, the
-// first chunk's CR + 2 rows, rows offset by up to VE - 1 elements)
-template <int S>
+// first chunk's CR + 2 rows, ranges offset by up to VE - 1 elements)
+template <int L, int S>
 constexpr uint32_t tile_staged_vectors()
 {
     constexpr uint32_t VE = 16u / sizeof(typename Storage<S>::T), NT = TILE_STAGED_THREADS;
+    constexpr uint32_t K = tile_staged_k(L), NRG = 9u / K;
     uint32_t kv = 0;
     for (uint32_t sh = TILE_STAGED_MIN_WIDTH; sh <= TILE_SWEEP_MAX_WIDTH; ++sh)
     {
-        const uint32_t rows = tile_staged_npt<S>() * (NT / sh) + 2u;
-        const uint32_t nv = (VE - 1u + 9u * (sh + 2u) + VE - 1u) / VE;
-        const uint32_t k = (rows * nv + NT - 1u) / NT;
+        const uint32_t rows = NT / sh + 2u;
+        const uint32_t nvs = (VE - 1u + K * (sh + 2u) + VE - 1u) / VE;
+        const uint32_t k = (rows * NRG * nvs + NT - 1u) / NT;
         kv = k > kv ? k : kv;
     }
     return kv;
@@ -893,69 +891,79 @@ constexpr uint32_t tile_staged_vectors()
 
 // side-array values of one chunk, at most: six per staged row (three for
 // each buffer column), 3 (sub_h + 2) for the buffer row below and above
-constexpr bool tile_staged_side_fits(uint32_t npt)
+constexpr bool tile_staged_side_fits()
 {
     for (uint32_t sh = TILE_STAGED_MIN_WIDTH; sh <= TILE_SWEEP_MAX_WIDTH; ++sh)
-        if (6u * (npt * (TILE_STAGED_THREADS / sh) + 2u) + 6u * (sh + 2u) > TILE_STAGED_SIDE * TILE_STAGED_THREADS)
+        if (6u * (TILE_STAGED_THREADS / sh + 2u) + 6u * (sh + 2u) > TILE_STAGED_SIDE * TILE_STAGED_THREADS)
             return false;
     return true;
 }
-static_assert(tile_staged_side_fits(tile_staged_npt_of(2)) && tile_staged_side_fits(tile_staged_npt_of(4)) &&
-                  tile_staged_side_fits(tile_staged_npt_of(8)),
-              "side values of a chunk exceed the per-thread slots");
+static_assert(tile_staged_side_fits(), "side values of a chunk exceed the per-thread slots");
 
-// Elements of a ring slot and of an output row, and the dynamic LDS bytes
-// (ring, output rows, two sets of fluid masks).
-__host__ __device__ inline uint32_t tile_staged_rowlen(uint32_t sh, uint32_t ve)
+// Elements of one range of a ring slot and of an output row (16-byte
+// multiples), and the dynamic LDS bytes (ring, output rows, two sets of
+// fluid masks).
+__host__ __device__ inline uint32_t tile_staged_sublen(uint32_t k, uint32_t sh, uint32_t ve)
 {
-    return (9u * (sh + 2u) + 2u * ve - 2u) / ve * ve;
+    return (k * (sh + 2u) + 2u * ve - 2u) / ve * ve;
 }
-__host__ __device__ inline uint32_t tile_staged_outlen(uint32_t sh, uint32_t ve)
+__host__ __device__ inline uint32_t tile_staged_outsub(uint32_t k, uint32_t sh, uint32_t ve)
 {
-    return (9u * sh + 2u * ve - 2u) / ve * ve;
+    return (k * sh + 2u * ve - 2u) / ve * ve;
 }
-inline size_t tile_staged_lds(uint32_t sh, size_t elem)
+inline size_t tile_staged_lds(int layout, uint32_t sh, size_t elem)
 {
-    const uint32_t ve = uint32_t(16 / elem), cr = tile_staged_npt_of(elem) * (TILE_STAGED_THREADS / sh);
-    return (size_t(cr + 2u) * tile_staged_rowlen(sh, ve) + size_t(cr) * tile_staged_outlen(sh, ve)) * elem +
+    const uint32_t ve = uint32_t(16 / elem), cr = TILE_STAGED_THREADS / sh;
+    const uint32_t k = tile_staged_k(layout), nrg = 9u / k;
+    return (size_t(cr + 2u) * nrg * tile_staged_sublen(k, sh, ve) + size_t(cr) * nrg * tile_staged_outsub(k, sh, ve)) *
+               elem +
            2u * size_t(cr) * sizeof(unsigned long long);
 }
 
-template <int S>
+template <int L, int S>
 __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepParams p)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
     using V4 = unsigned int __attribute__((ext_vector_type(4)));
-    constexpr int L = LBM_LAYOUT_COLLISION;
-    constexpr uint32_t NT = TILE_STAGED_THREADS, VE = 16u / sizeof(T), KV = tile_staged_vectors<S>();
-    constexpr uint32_t KS = TILE_STAGED_SIDE, NPT = tile_staged_npt<S>();
-    constexpr uint32_t SE = 64u / sizeof(T);  // elements per 64-byte segment
+    constexpr uint32_t NT = TILE_STAGED_THREADS, VE = 16u / sizeof(T), KV = tile_staged_vectors<L, S>();
+    constexpr uint32_t KS = TILE_STAGED_SIDE;
+    constexpr uint32_t K = tile_staged_k(L), NRG = 9u / K;  // elements per node and range, ranges per row
+    constexpr uint32_t SE = 64u / sizeof(T);                 // elements per 64-byte segment
     extern __shared__ __align__(16) unsigned char tile_lds[];
     const uint32_t sh = p.sub_h, sv = p.sub_v, P = p.P, Hx = p.Hx, Vx = p.Vx;
-    const uint32_t R1 = NT / sh, CR = NPT * R1, W = CR + 2u;  // node rows per thread row pass, per chunk
-    const uint32_t rowlen = tile_staged_rowlen(sh, VE), outlen = tile_staged_outlen(sh, VE);
+    const uint32_t CR = NT / sh, W = CR + 2u;
+    const uint32_t sub = tile_staged_sublen(K, sh, VE), osub = tile_staged_outsub(K, sh, VE);
+    const uint32_t rowl = NRG * sub, outl = NRG * osub;  // elements of a ring slot, of an output row
+    // element distance between a row's ranges in the lattice
+    const uint32_t rsl = L == LBM_LAYOUT_STREAM ? p.S : (L == LBM_LAYOUT_BUNDLE ? 3u * p.S : 0u);
     T *ring = reinterpret_cast<T *>(tile_lds);
-    T *out = ring + W * rowlen;
-    unsigned long long *masks = reinterpret_cast<unsigned long long *>(out + CR * outlen);  // [2][CR]
+    T *out = ring + W * rowl;
+    unsigned long long *masks = reinterpret_cast<unsigned long long *>(out + CR * outl);  // [2][CR]
     T *lat = static_cast<T *>(p.dst);
     const uint32_t tid = threadIdx.x, ty = tid / sh, tx = tid - ty * sh;
     const uint32_t bx = blockIdx.x, by = blockIdx.y;
     const uint32_t x0 = bx * (sh + 1u) + 1u, y00 = by * (sv + 1u) + 1u;
     const bool buf_l = bx > 0u, buf_r = bx + 1u < p.cnt_h;  // columns x0 - 1, x0 + sh are buffers
     const bool buf_b = by > 0u, buf_t = by + 1u < p.cnt_v;  // rows y00 - 1, y00 + sv are buffers
-    // window row j (0 .. sv + 1) is lattice row y00 - 1 + j; its elements
-    // start at e00 + 9 j P, every row at the same offset mod VE (P is a
-    // multiple of 64 places); place i of a row is column x0 - 1 + i
-    const uint32_t e00 = 9u * place(x0 - 1u, y00 - 1u, P, p.X0);
+    // window row j (0 .. sv + 1) is lattice row y00 - 1 + j; range a of it
+    // starts at element a rsl + e00 + K j P, every range of every row at the
+    // same offset mod VE (P is a multiple of 64 places, rsl of 16 bytes);
+    // place i of a row is column x0 - 1 + i
+    const uint32_t e00 = K * place(x0 - 1u, y00 - 1u, P, p.X0);
     const uint32_t shift = e00 % VE, a00 = e00 - shift;
-    const uint32_t nv = (shift + 9u * (sh + 2u) + VE - 1u) / VE;  // vectors per row
-    const uint32_t s1 = (e00 + 9u) % VE;                          // offset of the tile's first node mod VE
-    const uint32_t nvo = (s1 + 9u * sh + VE - 1u) / VE;           // output vectors per row
+    const uint32_t nvs = (shift + K * (sh + 2u) + VE - 1u) / VE;  // vectors per range
+    const uint32_t nv = NRG * nvs;                                // per row
+    const uint32_t s1 = (e00 + K) % VE;                           // offset of the tile's first node mod VE
+    const uint32_t nvo = (s1 + K * sh + VE - 1u) / VE;            // output vectors per range
     const uint32_t chunks = (sv + CR - 1u) / CR;
     const uint32_t xe = x0 + tx, xu = xe - xe / (sh + 1u);
     const Rt3 r3 = rt3_of<S>(p.ref_in, xu);
     const float rto = rt_of<S>(p.ref_out, xu) - r3.own();
+    // ring element of population d at place i of window row j
+    auto slot = [&](uint32_t j, uint32_t i, uint32_t d) -> uint32_t {
+        return (j % W) * rowl + (d / K) * sub + shift + K * i + d % K;
+    };
 
     V4 pre[KV];
     T side[KS];
@@ -970,8 +978,8 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
         {
             const uint32_t w = tid + k * NT;
             const uint32_t wc = w < total ? w : 0u;
-            const uint32_t r = wc / nv, v = wc - r * nv;
-            pre[k] = *reinterpret_cast<const V4 *>(lat + (a00 + 9u * (ja + r) * P + v * VE));
+            const uint32_t r = wc / nv, rem = wc - r * nv, ra = rem / nvs, v = rem - ra * nvs;
+            pre[k] = *reinterpret_cast<const V4 *>(lat + (ra * rsl + a00 + K * (ja + r) * P + v * VE));
         }
         // buffer-column places of rows that are not buffer rows: populations
         // 2, 5, 8 on the left, 0, 3, 6 on the right; buffer rows: 6, 7, 8
@@ -993,7 +1001,7 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
                 {
                     const uint32_t kcol = s == 0u ? (bx - 1u) * 6u + q : bx * 6u + 3u + q;
                     side[k] = lat[p.side_cols_in + kcol * Vx + (y00 - 1u + j)];
-                    side_at[k] = (j % W) * rowlen + shift + (s == 0u ? 3u * q + 2u : 9u * (sh + 1u) + 3u * q);
+                    side_at[k] = s == 0u ? slot(j, 0u, 3u * q + 2u) : slot(j, sh + 1u, 3u * q);
                 }
             }
             else if (w < nc + nb + nt)
@@ -1002,9 +1010,8 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
                 const uint32_t q = v / (sh + 2u), i = v - q * (sh + 2u);
                 const uint32_t xs = (i == 0u && buf_l) ? x0 - 2u : ((i == sh + 1u && buf_r) ? x0 + sh + 1u : x0 - 1u + i);
                 const uint32_t krow = s == 0u ? (by - 1u) * 6u + 3u + q : by * 6u + q;
-                const uint32_t j = s == 0u ? 0u : sv + 1u;
                 side[k] = lat[p.side_rows_in + krow * Hx + xs];
-                side_at[k] = (j % W) * rowlen + shift + 9u * i + (s == 0u ? 6u + q : q);
+                side_at[k] = s == 0u ? slot(0u, i, 6u + q) : slot(sv + 1u, i, q);
             }
         }
     };
@@ -1016,8 +1023,8 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
             const uint32_t w = tid + k * NT;
             if (w < total)
             {
-                const uint32_t r = w / nv, v = w - r * nv;
-                reinterpret_cast<V4 *>(ring + ((ja + r) % W) * rowlen)[v] = pre[k];
+                const uint32_t r = w / nv, rem = w - r * nv, ra = rem / nvs, v = rem - ra * nvs;
+                reinterpret_cast<V4 *>(ring + ((ja + r) % W) * rowl + ra * sub)[v] = pre[k];
             }
         }
     };
@@ -1027,16 +1034,12 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
             if (side_at[k] != ~0u)
                 ring[side_at[k]] = side[k];
     };
-    // this thread's node h of chunk k is in window row 1 + k CR + ty + h R1
-    auto code_of = [&](uint32_t k, uint32_t h) -> uint32_t {
-        const uint32_t j = 1u + k * CR + ty + h * R1;
-        return (ty < R1 && j <= sv) ? uint32_t(p.code[place(xe, y00 - 1u + j, P, p.X0)]) : 0u;
+    auto code_of = [&](uint32_t k) -> uint32_t {
+        const uint32_t j = 1u + k * CR + ty;
+        return (ty < CR && j <= sv) ? uint32_t(p.code[place(xe, y00 - 1u + j, P, p.X0)]) : 0u;
     };
 
-    uint32_t code[NPT];
-#pragma unroll
-    for (uint32_t h = 0; h < NPT; ++h)
-        code[h] = code_of(0, h);
+    uint32_t code = code_of(0);
     fetch(0u, min(CR + 1u, sv + 1u));
     if (tid < 2u * CR)
         masks[tid] = 0ull;
@@ -1048,109 +1051,88 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
     {
         const bool more = k + 1u < chunks;
         const uint32_t na = (k + 1u) * CR + 2u, nb = min((k + 2u) * CR + 1u, sv + 1u);  // next chunk's new rows
-        uint32_t code_next[NPT] = {};
+        uint32_t code_next = 0;
         if (more)
         {
-#pragma unroll
-            for (uint32_t h = 0; h < NPT; ++h)
-                code_next[h] = code_of(k + 1u, h);
+            code_next = code_of(k + 1u);
             fetch(na, nb);
         }
         unsigned long long *fluid_row = masks + (k & 1u) * CR;
-        bool any_odd = false;
-        T t[NPT][9];
-#pragma unroll
-        for (uint32_t h = 0; h < NPT; ++h)
+        const uint32_t j = 1u + k * CR + ty;  // window row of this thread's node
+        const bool active = ty < CR && j <= sv;
+        const uint32_t y = y00 - 1u + j;
+        const uint32_t pn = place(xe, y, P, p.X0);
+        const bool fluid = code & LBM_CODE_FLUID, solid = code & LBM_CODE_SOLID;
+        C f[9] = {};
+        if (fluid)
         {
-            const uint32_t rr = ty + h * R1;  // node row within the chunk
-            const uint32_t j = 1u + k * CR + rr;  // window row of this node
-            const bool active = ty < R1 && j <= sv;
-            const uint32_t y = y00 - 1u + j;
-            const uint32_t pn = place(xe, y, P, p.X0);
-            const uint32_t cd = code[h];
-            const bool fluid = cd & LBM_CODE_FLUID, solid = cd & LBM_CODE_SOLID;
-            any_odd |= active && !fluid && !solid;
-            C f[9] = {};
-            if (fluid)
-            {
 #pragma unroll
-                for (int d = 0; d < 9; ++d)
-                {
-                    const bool bb = (cd >> d) & 1u;
-                    const uint32_t jr = bb ? j : uint32_t(int32_t(j) - cy_of(d));
-                    const uint32_t i = bb ? tx + 1u : uint32_t(int32_t(tx) + 1 - cx_of(d));
-                    const int dd = bb ? 8 - d : d;
-                    f[d] = Storage<S>::load(ring + (jr % W) * rowlen + shift, 9u * i + uint32_t(dd), dd,
-                                            bb ? 0.f : r3.rel(cx_of(d)));
-                }
-                finish_node<L, S>(p, cd, xu, y - y / (sv + 1u), f, r3.own());
-                atomicOr(&fluid_row[rr], 1ull << tx);
-            }
-            node_bits<S>(f, pn, p.sr, t[h], rto);
-            if (fluid)
+            for (int d = 0; d < 9; ++d)
             {
-                // the new edge populations the neighbouring tiles pull next step
-                if (tx == sh - 1u && buf_r)
-                    for (uint32_t q = 0; q < 3; ++q)
-                        lat[p.side_cols_out + (bx * 6u + q) * Vx + y] = t[h][3u * q + 2u];
-                if (tx == 0u && buf_l)
-                    for (uint32_t q = 0; q < 3; ++q)
-                        lat[p.side_cols_out + ((bx - 1u) * 6u + 3u + q) * Vx + y] = t[h][3u * q];
-                if (j == sv && buf_t)
-                    for (uint32_t q = 0; q < 3; ++q)
-                        lat[p.side_rows_out + (by * 6u + 3u + q) * Hx + xe] = t[h][6u + q];
-                if (j == 1u && buf_b)
-                    for (uint32_t q = 0; q < 3; ++q)
-                        lat[p.side_rows_out + ((by - 1u) * 6u + q) * Hx + xe] = t[h][q];
+                const bool bb = (code >> d) & 1u;
+                const uint32_t jr = bb ? j : uint32_t(int32_t(j) - cy_of(d));
+                const uint32_t i = bb ? tx + 1u : uint32_t(int32_t(tx) + 1 - cx_of(d));
+                const int dd = bb ? 8 - d : d;
+                f[d] = Storage<S>::load(ring, slot(jr, i, uint32_t(dd)), dd, bb ? 0.f : r3.rel(cx_of(d)));
             }
+            finish_node<L, S>(p, code, xu, y - y / (sv + 1u), f, r3.own());
+            atomicOr(&fluid_row[ty], 1ull << tx);
+        }
+        T t[9];
+        node_bits<S>(f, pn, p.sr, t, rto);
+        if (fluid)
+        {
+            // the new edge populations the neighbouring tiles pull next step
+            if (tx == sh - 1u && buf_r)
+                for (uint32_t q = 0; q < 3; ++q)
+                    lat[p.side_cols_out + (bx * 6u + q) * Vx + y] = t[3u * q + 2u];
+            if (tx == 0u && buf_l)
+                for (uint32_t q = 0; q < 3; ++q)
+                    lat[p.side_cols_out + ((bx - 1u) * 6u + 3u + q) * Vx + y] = t[3u * q];
+            if (j == sv && buf_t)
+                for (uint32_t q = 0; q < 3; ++q)
+                    lat[p.side_rows_out + (by * 6u + 3u + q) * Hx + xe] = t[6u + q];
+            if (j == 1u && buf_b)
+                for (uint32_t q = 0; q < 3; ++q)
+                    lat[p.side_rows_out + ((by - 1u) * 6u + q) * Hx + xe] = t[q];
         }
         // every pull of the chunk is done: the ring slots of rows below j - 1
         // may take the next chunk's rows
-        const bool odd = __syncthreads_or(any_odd);
-#pragma unroll
-        for (uint32_t h = 0; h < NPT; ++h)
+        const bool odd = __syncthreads_or(active && !fluid && !solid);
+        if (odd)
         {
-            const uint32_t rr = ty + h * R1, j = 1u + k * CR + rr;
-            if (!(ty < R1 && j <= sv))
-                continue;
-            if (odd)
-            {
-                const uint32_t cd = code[h];
-                if (cd & (LBM_CODE_FLUID | LBM_CODE_SOLID))
-                {
-                    const uint32_t pn = place(xe, y00 - 1u + j, P, p.X0);
-#pragma unroll
-                    for (int d = 0; d < 9; ++d)
-                        lat[9u * pn + uint32_t(d)] = t[h][d];
-                }
-            }
-            else
-            {
+            if (fluid || solid)
 #pragma unroll
                 for (int d = 0; d < 9; ++d)
-                    out[rr * outlen + s1 + 9u * tx + uint32_t(d)] = t[h][d];
-            }
+                    lat[at<L>(pn, uint32_t(d), p.S)] = t[d];
+        }
+        else if (active)
+        {
+#pragma unroll
+            for (int d = 0; d < 9; ++d)
+                out[ty * outl + uint32_t(d) / K * osub + s1 + K * tx + uint32_t(d) % K] = t[d];
         }
         __syncthreads();
         if (!odd)
         {
             const uint32_t rows = min(CR, sv - k * CR);
-            for (uint32_t w = tid; w < rows * nvo; w += NT)
+            for (uint32_t w = tid; w < rows * NRG * nvo; w += NT)
             {
-                const uint32_t r = w / nvo, v = w - r * nvo;
-                const uint32_t g0 = e00 + 9u + 9u * (1u + k * CR + r) * P;  // the row's first tile element
-                const uint32_t ga = g0 - s1 + v * VE;                       // 16-byte aligned
+                const uint32_t r = w / (NRG * nvo), rem = w - r * (NRG * nvo), ra = rem / nvo, v = rem - ra * nvo;
+                // the range's first tile element
+                const uint32_t g0 = ra * rsl + e00 + K + K * (1u + k * CR + r) * P;
+                const uint32_t ga = g0 - s1 + v * VE;  // 16-byte aligned
                 const uint32_t seg = ga / SE * SE;
-                const uint32_t lo = max(seg, g0), hi = min(seg + SE, g0 + 9u * sh) - 1u;
-                const uint32_t n0 = (lo - g0) / 9u, n = (hi - g0) / 9u - n0 + 1u;  // n <= SE / 9 + 2 < 64
+                const uint32_t lo = max(seg, g0), hi = min(seg + SE, g0 + K * sh) - 1u;
+                const uint32_t n0 = (lo - g0) / K, n = (hi - g0) / K - n0 + 1u;  // n <= SE / K + 2 < 64
                 if (!((fluid_row[r] >> n0) & ((1ull << n) - 1ull)))
                     continue;
-                const T *o = out + r * outlen + v * VE;
-                if (v * VE >= s1 && v * VE + VE <= s1 + 9u * sh)
+                const T *o = out + r * outl + ra * osub + v * VE;
+                if (v * VE >= s1 && v * VE + VE <= s1 + K * sh)
                     __builtin_nontemporal_store(*reinterpret_cast<const V4 *>(o), reinterpret_cast<V4 *>(lat + ga));
                 else
                     for (uint32_t e = 0; e < VE; ++e)
-                        if (v * VE + e >= s1 && v * VE + e < s1 + 9u * sh)
+                        if (v * VE + e >= s1 && v * VE + e < s1 + K * sh)
                             lat[ga + e] = o[e];
             }
         }
@@ -1163,9 +1145,7 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
             __syncthreads();
             patch();
         }
-#pragma unroll
-        for (uint32_t h = 0; h < NPT; ++h)
-            code[h] = code_next[h];
+        code = code_next;
         __syncthreads();
     }
 }
@@ -1665,14 +1645,11 @@ struct Impl
             c->side_parity ^= 1;
         };
         auto launch_tile_staged = [&]() {
-            if constexpr (L == LBM_LAYOUT_COLLISION)
-            {
-                if (!c->side_valid)
-                    prime(c);
-                hipLaunchKernelGGL((k_step_tile_staged<S>), dim3(c->cnt_h, c->cnt_v), dim3(TILE_STAGED_THREADS),
-                                   tile_staged_lds(c->sub_h, sizeof(T)), c->stream, p);
-                c->side_parity ^= 1;
-            }
+            if (!c->side_valid)
+                prime(c);
+            hipLaunchKernelGGL((k_step_tile_staged<L, S>), dim3(c->cnt_h, c->cnt_v), dim3(TILE_STAGED_THREADS),
+                               tile_staged_lds(L, c->sub_h, sizeof(T)), c->stream, p);
+            c->side_parity ^= 1;
         };
         switch (c->desc.algorithm)
         {
@@ -2158,12 +2135,15 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out)
     // 256, 248 tiles, 17.7 against 18.6 us).  In 16-bit storage the sweep
     // kept its place: staged 36.5 / 37.5 against 35.9 / 37.4 GFLUPS (nsol /
     // npol FP16 on the porous 8192 x 2048 channel), its side arrays costing
-    // 2-byte scattered loads and stores.
+    // 2-byte scattered loads and stores.  So did the stream and bundle
+    // layouts, whose nodes the sweep's lanes read contiguously: staged FP64
+    // npol stream 14.4 against 19.0, nsol bundle 13.6 against 21.3; FP32
+    // 21.2 / 26.5 against 33.1 / 33.0; FP16 26.5 / 28.8 against 48.7 / 43.5.
     const uint64_t tiles = uint64_t(c->cnt_h) * c->cnt_v;
     const bool auto_staged = c->elem >= 4 && tiles >= uint64_t(std::max(cus, 1)) / (c->elem == 8 ? 2u : 1u);
-    c->tile_staged = c->buffered && desc->layout == LBM_LAYOUT_COLLISION && c->sub_h >= TILE_STAGED_MIN_WIDTH &&
-                     c->sub_h <= TILE_SWEEP_MAX_WIDTH &&
-                     (desc->tile_kernel == LBM_TILE_STAGED || (desc->tile_kernel == LBM_TILE_AUTO && auto_staged));
+    c->tile_staged = c->buffered && c->sub_h >= TILE_STAGED_MIN_WIDTH && c->sub_h <= TILE_SWEEP_MAX_WIDTH &&
+                     (desc->tile_kernel == LBM_TILE_STAGED ||
+                      (desc->tile_kernel == LBM_TILE_AUTO && auto_staged && desc->layout == LBM_LAYOUT_COLLISION));
     const bool auto_sweep =
         !few_tiles && !(desc->algorithm == LBM_ALG_NPOL && desc->layout == LBM_LAYOUT_COLLISION && c->elem == 8);
     const bool want_sweep = !c->tile_staged && c->buffered && c->sub_h <= TILE_SWEEP_MAX_WIDTH &&

@@ -56,19 +56,11 @@ RAGGED = [
 
 
 # npol/nsol shapes run with every step kernel of tiles at most a wave wide:
-# the one-wave sweep, the block kernel with buffer refreshes and, in the
-# collision layout, the staged block kernel (tiles 8 to 64 nodes wide; the
-# others run the automatic choice there)
-
-
-def _kernels(alg, layout):
-    if alg not in ("npol", "nsol"):
-        return ("auto",)
-    return ("sweep", "block", "staged") if layout == "collision" else ("sweep", "block")
-
-
+# the one-wave sweep, the block kernel with buffer refreshes and the staged
+# block kernel (tiles 8 to 64 nodes wide; the others run the automatic
+# choice there)
 RAGGED_KERNELS = [(layout, a, w, h, wg, t) for layout in LAYOUTS for a, w, h, wg in RAGGED
-                  for t in _kernels(a, layout)]
+                  for t in (("sweep", "block", "staged") if a in ("npol", "nsol") else ("auto",))]
 
 
 @pytest.mark.parametrize("layout,alg,w,h,wg,tile", RAGGED_KERNELS)

@@ -6,8 +6,8 @@ densities), with the collision computed on the deviations in FP32.
 * Every algorithm, layout and kernel decodes a population with the reference
   of the column that holds it: with round-to-nearest the four algorithms give
   the same fields bit for bit (row kernel, staged kernel, lptl's 1-D kernel,
-  npol/nsol's tile sweep and their block-per-tile kernels for tiles wider
-  than a wave), also where buffer columns separate the tiles.
+  npol/nsol's tile sweep, their staged block kernel and their block-per-tile
+  kernels), also where buffer columns separate the tiles.
 * The same holds on tiny and ragged lattices, one-row and wide tiles and the
   buffered-outlet shapes.
 * The accuracy this buys: on a 1024 x 256 porous channel after 5000 steps
@@ -44,7 +44,7 @@ CASES = [("lptl", 256), ("npol", 256), ("npol", 1024), ("npol", 130), ("nsol", 2
 def test_sixteen_bit_fields_equal_across_algorithms(storage, layout):
     ref = _fields("nptl", layout, storage, 256)
     for alg, wg in CASES:
-        for tile in (("sweep", "block") if alg in ("npol", "nsol") else ("auto",)):
+        for tile in (("sweep", "block", "staged") if alg in ("npol", "nsol") else ("auto",)):
             got = _fields(alg, layout, storage, wg, tile=tile)
             for name, a, b in zip(("rho", "jx", "jy", "|j|"), got, ref):
                 assert np.array_equal(a, b), (alg, wg, tile, layout, storage, name, float(np.max(np.abs(a - b))))
@@ -69,7 +69,7 @@ def _outlet_quirk(alg, w, h, wg):
 
 
 RAGGED16_KERNELS = [(a, w, h, wg, t) for a, w, h, wg in RAGGED16
-                    for t in (("sweep", "block") if a in ("npol", "nsol") else ("auto",))]
+                    for t in (("sweep", "block", "staged") if a in ("npol", "nsol") else ("auto",))]
 
 
 @pytest.mark.parametrize("layout", ["collision", "stream", "bundle"])
