@@ -122,6 +122,9 @@ def hip_library() -> ctypes.CDLL:
             "lbm_reduce": ([_P, _P], ctypes.c_int),
             "lbm_residual": ([_P, _DP], ctypes.c_int),
             "lbm_sync": ([_P], ctypes.c_int),
+            "lbm_state_size": ([_P, ctypes.POINTER(_SZ)], ctypes.c_int),
+            "lbm_get_state": ([_P, _P, _SZ], ctypes.c_int),
+            "lbm_set_state": ([_P, _P, _SZ], ctypes.c_int),
             "lbm_measure_copy_bandwidth": ([_SZ, ctypes.c_int, _DP], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
@@ -398,6 +401,18 @@ class Lattice:
     def set_populations(self, values: np.ndarray) -> None:
         values = np.ascontiguousarray(values, dtype=np.float64)
         _check_hip(self.lib.lbm_set_populations(self._ctx, _ptr(values), values.size), "lbm_set_populations")
+
+    def state(self) -> np.ndarray:
+        """The whole lattice state (lbm_get_state) as bytes, for set_state."""
+        n = _SZ(0)
+        _check_hip(self.lib.lbm_state_size(self._ctx, ctypes.byref(n)), "lbm_state_size")
+        out = np.empty(n.value, dtype=np.uint8)
+        _check_hip(self.lib.lbm_get_state(self._ctx, _ptr(out), out.size), "lbm_get_state")
+        return out
+
+    def set_state(self, state: np.ndarray) -> None:
+        state = np.ascontiguousarray(state, dtype=np.uint8)
+        _check_hip(self.lib.lbm_set_state(self._ctx, _ptr(state), state.size), "lbm_set_state")
 
     def run_split(self, steps: int, callback=None) -> float:
         """Split (debug) steps; callback(pass, iteration) after every pass."""

@@ -2512,16 +2512,17 @@ int lbm_get_state(lbm_ctx *c, void *buf, size_t bytes)
     std::memcpy(out + 12, &lattices, 4);
     size_t off = 16;
     const size_t lat_bytes = c->lattice_elems * c->elem;
+    // the caller's buffer is pageable host memory: blocking copies, after
+    // the work on the context's stream (non-blocking) has finished
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
     for (uint32_t l = 0; l < lattices; ++l, off += lat_bytes)
-        LBM_HIP_TRY(hipMemcpyAsync(out + off, c->lat[l], lat_bytes, hipMemcpyDeviceToHost, c->stream));
-    LBM_HIP_TRY(hipMemcpyAsync(out + off, c->uycol, 2ull * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        LBM_HIP_TRY(hipMemcpy(out + off, c->lat[l], lat_bytes, hipMemcpyDeviceToHost));
+    LBM_HIP_TRY(hipMemcpy(out + off, c->uycol, 2ull * c->V * sizeof(double), hipMemcpyDeviceToHost));
     off += 2ull * c->V * sizeof(double);
-    LBM_HIP_TRY(hipMemcpyAsync(out + off, c->res, 4ull * c->D * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LBM_HIP_TRY(hipMemcpy(out + off, c->res, 4ull * c->D * sizeof(double), hipMemcpyDeviceToHost));
     off += 4ull * c->D * sizeof(double);
     if (c->ref_len)
-        LBM_HIP_TRY(hipMemcpyAsync(out + off, c->ref[c->ref_cur], c->ref_len * sizeof(float), hipMemcpyDeviceToHost,
-                                   c->stream));
-    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+        LBM_HIP_TRY(hipMemcpy(out + off, c->ref[c->ref_cur], c->ref_len * sizeof(float), hipMemcpyDeviceToHost));
     return LBM_OK;
 }
 
@@ -2543,17 +2544,20 @@ int lbm_set_state(lbm_ctx *c, const void *buf, size_t bytes)
         return fail(LBM_ERR_INVALID_ARGUMENT, "lbm_set_state: state of another algorithm");
     size_t off = 16;
     const size_t lat_bytes = c->lattice_elems * c->elem;
+    // pageable source: blocking copies once the context's stream is idle,
+    // and a device-wide wait before the next launch reads the data
+    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
     for (uint32_t l = 0; l < lattices; ++l, off += lat_bytes)
-        LBM_HIP_TRY(hipMemcpyAsync(c->lat[l], in + off, lat_bytes, hipMemcpyHostToDevice, c->stream));
-    LBM_HIP_TRY(hipMemcpyAsync(c->uycol, in + off, 2ull * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        LBM_HIP_TRY(hipMemcpy(c->lat[l], in + off, lat_bytes, hipMemcpyHostToDevice));
+    LBM_HIP_TRY(hipMemcpy(c->uycol, in + off, 2ull * c->V * sizeof(double), hipMemcpyHostToDevice));
     off += 2ull * c->V * sizeof(double);
-    LBM_HIP_TRY(hipMemcpyAsync(c->res, in + off, 4ull * c->D * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    LBM_HIP_TRY(hipMemcpy(c->res, in + off, 4ull * c->D * sizeof(double), hipMemcpyHostToDevice));
     off += 4ull * c->D * sizeof(double);
     c->ref_cur = 0;
     c->ref_switch = false;
     if (c->ref_len)
-        LBM_HIP_TRY(hipMemcpyAsync(c->ref[0], in + off, c->ref_len * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    LBM_HIP_TRY(hipStreamSynchronize(c->stream));
+        LBM_HIP_TRY(hipMemcpy(c->ref[0], in + off, c->ref_len * sizeof(float), hipMemcpyHostToDevice));
+    LBM_HIP_TRY(hipDeviceSynchronize());
     c->iteration = iteration;
     c->frame_iteration = ~0ull;
     c->cur = int(cur);
