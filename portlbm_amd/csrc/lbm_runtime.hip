@@ -194,8 +194,24 @@ __global__ __launch_bounds__(NPTL_BLOCK_X) void k_step_nptl(StepParams p)
 // 88.8 GFLUPS in FP16: the bounce-back pulls of the six directions with
 // cy != 0 read rows that are not staged; staging the bundle layout 53.8, or
 // 61.9 with the flat copy, against 63.7 in FP16, and 52.4 / 58.8 against
-// 64.0 in BF16).
+// 64.0 in BF16).  The collision layout's row segments were then narrowed to
+// 32 nodes in FP64 and 16-bit storage (below).
 constexpr uint32_t NPTL_STAGED_BLOCK = 64;
+
+// Nodes per row segment of a block (a divisor of 64, so that segments start
+// on 64-byte boundaries and a wave holds whole rows).  32 x R blocks measured
+// faster than 64 x R in the FP64 and 16-bit collision layout on the porous
+// 8192 x 2048 channel (GFLUPS): FP64 32 x 8 32.2 against 64 x 8 31.9 (32 x
+// 16: 30.5); FP16 32 x 4 65.5 against 64 x 2 61.7 (32 x 2: 63.5, 32 x 8:
+// 59.9); FP32 32 x 4 and 64 x 4 49.2 both (32 x 8: 47.0).
+template <int L, int S>
+constexpr uint32_t nptl_staged_width()
+{
+    if constexpr (L == LBM_LAYOUT_COLLISION && S != LBM_STORE_F32)
+        return 32u;
+    else
+        return NPTL_STAGED_BLOCK;
+}
 
 template <int L, int S>
 constexpr bool nptl_staged()
@@ -228,19 +244,17 @@ constexpr uint32_t nptl_staged_rows()
         return 1u;
     else if constexpr (S == LBM_STORE_F64)
         return 8u;
-    else if constexpr (S == LBM_STORE_F32)
-        return 4u;
     else
-        return 2u;
+        return 4u;
 }
 
 template <int L, int S>
-__global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) void k_step_nptl_staged(StepParams p)
+__global__ __launch_bounds__((nptl_staged_width<L, S>() * nptl_staged_rows<L, S>())) void k_step_nptl_staged(StepParams p)
 {
     using T = typename Storage<S>::T;
     using C = typename Storage<S>::C;
     using V4 = unsigned int __attribute__((ext_vector_type(4)));
-    constexpr uint32_t B = NPTL_STAGED_BLOCK;
+    constexpr uint32_t B = nptl_staged_width<L, S>();
     constexpr uint32_t R = nptl_staged_rows<L, S>();               // node rows per block
     constexpr uint32_t NT = B * R;                                 // threads per block
     constexpr uint32_t VE = 16u / sizeof(T);                       // elements per 16-byte vector
@@ -380,11 +394,11 @@ __global__ __launch_bounds__((NPTL_STAGED_BLOCK * nptl_staged_rows<L, S>())) voi
         }
         finish_node<L, S>(p, code, x0 + tx, y, f, r3.own());
     }
-    // which places of each node row are fluid (a wave is a node row: B = 64)
+    // which places of each node row are fluid (a wave holds 64 / B whole rows)
     __shared__ unsigned long long fluid_row[R];
-    const unsigned long long row_mask = __ballot(fluid);
+    const unsigned long long row_mask = __ballot(fluid) >> ((ty * B) % 64u);
     if (tx == 0u)
-        fluid_row[ty] = row_mask;
+        fluid_row[ty] = B == 64u ? row_mask : row_mask & ((1ull << (B % 64u)) - 1ull);
     const bool odd = active && !fluid && !solid;
     if (__syncthreads_or(odd))
     {
@@ -965,6 +979,23 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
         return (j % W) * rowl + (d / K) * sub + shift + K * i + d % K;
     };
 
+    // row, range and vector of flat vector index w of a row batch (one range
+    // per row in the collision layout: no second division)
+    auto split = [&](uint32_t w, uint32_t per_row, uint32_t per_range, uint32_t &r, uint32_t &ra, uint32_t &v) {
+        r = w / per_row;
+        const uint32_t rem = w - r * per_row;
+        if constexpr (NRG == 1)
+        {
+            ra = 0u;
+            v = rem;
+        }
+        else
+        {
+            ra = rem / per_range;
+            v = rem - ra * per_range;
+        }
+    };
+
     V4 pre[KV];
     T side[KS];
     uint32_t side_at[KS];  // ring element of each side value, ~0u: none
@@ -978,7 +1009,8 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
         {
             const uint32_t w = tid + k * NT;
             const uint32_t wc = w < total ? w : 0u;
-            const uint32_t r = wc / nv, rem = wc - r * nv, ra = rem / nvs, v = rem - ra * nvs;
+            uint32_t r, ra, v;
+            split(wc, nv, nvs, r, ra, v);
             pre[k] = *reinterpret_cast<const V4 *>(lat + (ra * rsl + a00 + K * (ja + r) * P + v * VE));
         }
         // buffer-column places of rows that are not buffer rows: populations
@@ -1023,7 +1055,8 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
             const uint32_t w = tid + k * NT;
             if (w < total)
             {
-                const uint32_t r = w / nv, rem = w - r * nv, ra = rem / nvs, v = rem - ra * nvs;
+                uint32_t r, ra, v;
+                split(w, nv, nvs, r, ra, v);
                 reinterpret_cast<V4 *>(ring + ((ja + r) % W) * rowl + ra * sub)[v] = pre[k];
             }
         }
@@ -1118,7 +1151,8 @@ __global__ __launch_bounds__(TILE_STAGED_THREADS) void k_step_tile_staged(StepPa
             const uint32_t rows = min(CR, sv - k * CR);
             for (uint32_t w = tid; w < rows * NRG * nvo; w += NT)
             {
-                const uint32_t r = w / (NRG * nvo), rem = w - r * (NRG * nvo), ra = rem / nvo, v = rem - ra * nvo;
+                uint32_t r, ra, v;
+                split(w, NRG * nvo, nvo, r, ra, v);
                 // the range's first tile element
                 const uint32_t g0 = ra * rsl + e00 + K + K * (1u + k * CR + r) * P;
                 const uint32_t ga = g0 - s1 + v * VE;  // 16-byte aligned
@@ -1623,8 +1657,9 @@ struct Impl
             if constexpr (nptl_staged<L, S>())
             {
                 constexpr uint32_t rows = nptl_staged_rows<L, S>();
-                dim3 grid((c->Hx - 2u + NPTL_STAGED_BLOCK - 1u) / NPTL_STAGED_BLOCK, (c->Vx - 2u + rows - 1u) / rows);
-                hipLaunchKernelGGL((k_step_nptl_staged<L, S>), grid, dim3(NPTL_STAGED_BLOCK * rows), 0, c->stream, p);
+                constexpr uint32_t width = nptl_staged_width<L, S>();
+                dim3 grid((c->Hx - 2u + width - 1u) / width, (c->Vx - 2u + rows - 1u) / rows);
+                hipLaunchKernelGGL((k_step_nptl_staged<L, S>), grid, dim3(width * rows), 0, c->stream, p);
             }
         };
         auto launch_rows = [&]() {

@@ -47,11 +47,12 @@ def _node_of(e, layout, Sd):
     return r // 3, 3 * a + r % 3
 
 
-def _staged_rows(layout, storage):
-    """Node rows per block of k_step_nptl_staged (nptl_staged_rows)."""
+def _staged_shape(layout, storage):
+    """Nodes per row segment and node rows per block of k_step_nptl_staged
+    (nptl_staged_width, nptl_staged_rows)."""
     if layout != "collision":
-        return 1
-    return {"f64": 8, "f32": 4}.get(storage, 2)
+        return 64, 1
+    return {"f64": (32, 8), "f32": (64, 4)}.get(storage, (32, 4))
 
 
 def _check_nptl_staged(w, h, layout, storage, row_skew=0):
@@ -60,7 +61,7 @@ def _check_nptl_staged(w, h, layout, storage, row_skew=0):
     VE = 16 // elem
     P, X0, Sd = _placement(ext, elem)
     Hx, Vx = ext["Hx"], ext["Vx"]
-    B, R = 64, _staged_rows(layout, storage)
+    B, R = _staged_shape(layout, storage)
     K = 9 if layout == "collision" else 3
     bad = 0
     for by in range((Vx - 2 + R - 1) // R):
