@@ -251,9 +251,11 @@ int lbm_set_populations(lbm_ctx *ctx, const double *values, size_t count);
 /* The lattice state as the device holds it, for snapshots and restarts: a
  * 16-byte head (iteration u64, source-lattice index u32, lattice count u32),
  * every lattice in its storage type and physical placement, the outlet's uy
- * history (2 V doubles) and the four result arrays (4 (H-2)(V-2) doubles).
- * Loading it into a context created from the same description (and node
- * types) continues the run bit for bit.  No counterpart in the reference. */
+ * history (2 V doubles), the four result arrays (4 (H-2)(V-2) doubles) and,
+ * in the 16-bit forms, the column reference table (max(Hx, H) + 64 floats).
+ * buf is host memory; both calls return when the copy is complete.  Loading
+ * it into a context created from the same description (and node types)
+ * continues the run bit for bit.  No counterpart in the reference. */
 int lbm_state_size(const lbm_ctx *ctx, size_t *bytes);
 int lbm_get_state(lbm_ctx *ctx, void *buf, size_t bytes);
 int lbm_set_state(lbm_ctx *ctx, const void *buf, size_t bytes);
