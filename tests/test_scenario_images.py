@@ -12,8 +12,8 @@ grid, has to give the same obstacles:
   side, area within 5 %, overlap (IoU) at least 0.98 / 0.92 / 0.97
   (measured 0.986 / 0.931 / 0.983; the rest is the GUI's anti-aliased
   edges).  This is the only check of the wing's 1.4-power tail
-  (domain_initialization.hpp:178-189) that does not come from this build's
-  own reading;
+  (domain_initialization.hpp:178-189) that does not come from this
+  library's own reading of the stencil;
 * walls: the five one-node plates at the same pixel columns (within 1.5 px)
   with the same vertical extents (within 4 px);
 * Hagen-Poiseuille: no obstacle between the channel walls.
